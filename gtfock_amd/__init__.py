@@ -1,0 +1,404 @@
+"""MI355X-native GTFock Fock build and purification -- Python host mirror.
+
+The product is the C-ABI library ``gtfock_amd/_lib/libgtfock_hip.so`` (HIP
+kernels for gfx950 behind GTFock's ``PFock_*``, ``CInt_*`` and purif API).
+This module binds it with ctypes and mirrors the reference's call sequence
+(pscf/scf.c) in Python: the same names, argument meaning and status codes.
+It never computes anything itself and raises if the library is missing.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libgtfock_hip.so")
+DATA_DIR = os.path.join(_HERE, "data")
+CSRC_DIR = os.path.join(_HERE, "csrc")
+
+PFOCK_STATUS = {0: "SUCCESS", 1: "INIT_FAILED", 2: "ALLOC_FAILED", 3: "INVALID_VALUE",
+                4: "EXECUTION_FAILED", 5: "INTERNAL_ERROR"}
+MAT_D, MAT_F, MAT_J, MAT_K = 0, 1, 2, 3
+PAIR_TYPES = ["ss", "ps", "pp", "ds", "dp", "dd"]
+
+
+def class_names():
+    out = []
+    for a in range(6):
+        for b in range(a + 1):
+            out.append("(%s|%s)" % (PAIR_TYPES[a], PAIR_TYPES[b]))
+    return out
+
+
+def build(jobs=8):
+    """Compile libgtfock_hip.so for gfx950 with hipcc (make -C csrc)."""
+    subprocess.check_call(["make", "-s", "-j%d" % jobs, "-C", CSRC_DIR])
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+_vp = ctypes.c_void_p
+
+
+class GtfError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GtfError("libgtfock_hip.so is not built (%s); run gtfock_amd.build()" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "CInt_createBasisSet": ([ctypes.POINTER(_vp)], ctypes.c_int),
+        "CInt_destroyBasisSet": ([_vp], ctypes.c_int),
+        "CInt_loadBasisSet": ([_vp, ctypes.c_char_p, ctypes.c_char_p], ctypes.c_int),
+        "CInt_packBasisSet": ([_vp, ctypes.POINTER(_vp), _ip], ctypes.c_int),
+        "CInt_unpackBasisSet": ([_vp, _vp], ctypes.c_int),
+        "CInt_getNumShells": ([_vp], ctypes.c_int),
+        "CInt_getNumFuncs": ([_vp], ctypes.c_int),
+        "CInt_getNumAtoms": ([_vp], ctypes.c_int),
+        "CInt_getNumOccOrb": ([_vp], ctypes.c_int),
+        "CInt_getMaxShellDim": ([_vp], ctypes.c_int),
+        "CInt_getShellDim": ([_vp, ctypes.c_int], ctypes.c_int),
+        "CInt_getFuncStartInd": ([_vp, ctypes.c_int], ctypes.c_int),
+        "CInt_getFuncEndInd": ([_vp, ctypes.c_int], ctypes.c_int),
+        "CInt_getAtomStartInd": ([_vp, ctypes.c_int], ctypes.c_int),
+        "CInt_getTotalCharge": ([_vp], ctypes.c_int),
+        "CInt_getNneutral": ([_vp], ctypes.c_int),
+        "CInt_getNucEnergy": ([_vp], ctypes.c_double),
+        "CInt_getInitialGuess": ([_vp, ctypes.c_int, ctypes.POINTER(_dp), _ip, _ip], None),
+        "CInt_createERD": ([_vp, ctypes.POINTER(_vp), ctypes.c_int], ctypes.c_int),
+        "CInt_destroyERD": ([_vp], ctypes.c_int),
+        "CInt_computeShellQuartet": ([_vp, _vp, ctypes.c_int] + [ctypes.c_int] * 4 + [ctypes.POINTER(_dp), _ip],
+                                     ctypes.c_int),
+        "CInt_createOED": ([_vp, ctypes.POINTER(_vp)], ctypes.c_int),
+        "CInt_destroyOED": ([_vp], ctypes.c_int),
+        "CInt_computePairOvl": ([_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_dp), _ip], ctypes.c_int),
+        "CInt_computePairCoreH": ([_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_dp), _ip], ctypes.c_int),
+        "PFock_create": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                          ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+        "PFock_destroy": ([_vp], ctypes.c_int),
+        "PFock_setNumDenMat": ([ctypes.c_int, _vp], ctypes.c_int),
+        "PFock_putDenMat": ([ctypes.c_int] * 5 + [_dp, ctypes.c_int, _vp], ctypes.c_int),
+        "PFock_fillDenMat": ([ctypes.c_double, ctypes.c_int, _vp], ctypes.c_int),
+        "PFock_commitDenMats": ([_vp], ctypes.c_int),
+        "PFock_sync": ([_vp], ctypes.c_int),
+        "PFock_getMat": ([_vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_int] * 5 + [_dp], ctypes.c_int),
+        "PFock_computeFock": ([_vp, _vp], ctypes.c_int),
+        "PFock_createCoreHMat": ([_vp, _vp], ctypes.c_int),
+        "PFock_destroyCoreHMat": ([_vp], ctypes.c_int),
+        "PFock_getCoreHMat": ([_vp] + [ctypes.c_int] * 5 + [_dp], ctypes.c_int),
+        "PFock_createOvlMat": ([_vp, _vp], ctypes.c_int),
+        "PFock_destroyOvlMat": ([_vp], ctypes.c_int),
+        "PFock_getOvlMat": ([_vp] + [ctypes.c_int] * 5 + [_dp], ctypes.c_int),
+        "PFock_getOvlMat2": ([_vp] + [ctypes.c_int] * 5 + [_dp], ctypes.c_int),
+        "PFock_getMemorySize": ([_vp, _dp], ctypes.c_int),
+        "PFock_getStatistics": ([_vp], ctypes.c_int),
+        "gtf_pfock_stats": ([_vp, _dp], ctypes.c_int),
+        "gtf_pfock_csr": ([_vp, _ip, _ip, _ip, _dp], ctypes.c_int),
+        "gtf_pfock_class_counts": ([_vp, _lp], ctypes.c_int),
+        "gtf_pfock_quartets": ([_vp, _ip, ctypes.c_long], ctypes.c_long),
+        "gtf_pfock_device_ptrs": ([_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                   ctypes.POINTER(_vp)], ctypes.c_int),
+        "gtf_pfock_stream": ([_vp], _vp),
+        "gtf_pfock_device_bytes": ([_vp], ctypes.c_double),
+        "gtf_erd_quartets": ([_vp, ctypes.c_int, _ip, _dp], ctypes.c_int),
+        "gtf_dgemm_host": ([ctypes.c_int, _dp, _dp, _dp, ctypes.c_int], ctypes.c_int),
+        "create_purif": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], _vp),
+        "destroy_purif": ([_vp], None),
+        "compute_purification": ([_vp, _dp, _dp], ctypes.c_int),
+        "compute_diis": ([_vp, _vp, _dp, _dp, ctypes.c_int], None),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names the C headers under include/ declare (parsed from the headers)."""
+    import re
+    names = []
+    inc = os.path.join(os.path.dirname(_HERE), "include")
+    for h in sorted(os.listdir(inc)):
+        if not h.endswith(".h"):
+            continue
+        txt = open(os.path.join(inc, h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt):
+            n = m.group(1)
+            if n.startswith(("PFock_", "CInt_", "gtf_", "create_purif", "destroy_purif", "compute_")):
+                names.append(n)
+    return sorted(set(names))
+
+
+def _check(st, what):
+    if st != 0:
+        raise GtfError("%s failed: %s" % (what, PFOCK_STATUS.get(st, st)))
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+class BasisSet:
+    """CInt basis set (libcint/include/CInt.h:45-145)."""
+
+    def __init__(self, gbs, xyz):
+        L = lib()
+        self.h = _vp()
+        _check(L.CInt_createBasisSet(ctypes.byref(self.h)), "CInt_createBasisSet")
+        _check(L.CInt_loadBasisSet(self.h, gbs.encode(), xyz.encode()), "CInt_loadBasisSet")
+        self.nshells = L.CInt_getNumShells(self.h)
+        self.nbf = L.CInt_getNumFuncs(self.h)
+        self.natoms = L.CInt_getNumAtoms(self.h)
+        self.nocc = L.CInt_getNumOccOrb(self.h)
+        self.charge = L.CInt_getTotalCharge(self.h)
+        self.shell_dim = [L.CInt_getShellDim(self.h, s) for s in range(self.nshells)]
+        self.shell_fstart = [L.CInt_getFuncStartInd(self.h, s) for s in range(self.nshells)] + [self.nbf]
+
+    def nuc_energy(self):
+        return lib().CInt_getNucEnergy(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().CInt_destroyBasisSet(self.h)
+            self.h = None
+
+
+class PFock:
+    """The PFock engine (pfock/pfock.h:241-565) on one GPU."""
+
+    def __init__(self, basis, nprow=1, npcol=1, ntasks=4, tolscr=1e-11, max_numdmat=1, symm=1):
+        self.basis = basis
+        self.h = _vp()
+        _check(lib().PFock_create(basis.h, nprow, npcol, ntasks, tolscr, max_numdmat, symm,
+                                  ctypes.byref(self.h)), "PFock_create")
+        self.nbf = basis.nbf
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().PFock_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_num_den_mat(self, n):
+        return lib().PFock_setNumDenMat(n, self.h)
+
+    def put_den_mat(self, D, index=0):
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        n = self.nbf
+        _check(lib().PFock_putDenMat(0, n - 1, 0, n - 1, n, _p(D), index, self.h), "PFock_putDenMat")
+
+    def fill_den_mat(self, value, index=0):
+        _check(lib().PFock_fillDenMat(value, index, self.h), "PFock_fillDenMat")
+
+    def commit(self):
+        _check(lib().PFock_commitDenMats(self.h), "PFock_commitDenMats")
+
+    def compute_fock(self):
+        _check(lib().PFock_computeFock(self.basis.h, self.h), "PFock_computeFock")
+
+    def get_mat(self, mtype=MAT_F, index=0):
+        n = self.nbf
+        out = np.zeros((n, n))
+        _check(lib().PFock_getMat(self.h, mtype, index, 0, n - 1, 0, n - 1, n, _p(out)), "PFock_getMat")
+        return out
+
+    def fock(self, D, index=0):
+        """putDenMat + commit + computeFock + getMat(F): G(D) = 2J - K."""
+        self.put_den_mat(D, index)
+        self.commit()
+        self.compute_fock()
+        return self.get_mat(MAT_F, index)
+
+    def _one(self, create, get):
+        L = lib()
+        _check(getattr(L, create)(self.h, self.basis.h), create)
+        n = self.nbf
+        out = np.zeros((n, n))
+        _check(getattr(L, get)(self.h, 0, n - 1, 0, n - 1, n, _p(out)), get)
+        return out
+
+    def overlap(self):
+        return self._one("PFock_createOvlMat", "PFock_getOvlMat")
+
+    def x_matrix(self):
+        return self._one("PFock_createOvlMat", "PFock_getOvlMat2")
+
+    def core_hamiltonian(self):
+        return self._one("PFock_createCoreHMat", "PFock_getCoreHMat")
+
+    def stats(self):
+        out = np.zeros(8)
+        lib().gtf_pfock_stats(self.h, _p(out))
+        keys = ["usq", "uitl", "nnz", "maxvalue", "gpu_seconds", "wall_seconds", "create_seconds", "device_bytes"]
+        return dict(zip(keys, out.tolist()))
+
+    def csr(self):
+        ns = self.basis.nshells
+        ptr = np.zeros(ns + 1, dtype=np.int32)
+        cap = ns * ns
+        sid = np.zeros(cap, dtype=np.int32)
+        srid = np.zeros(cap, dtype=np.int32)
+        sval = np.zeros(cap)
+        nnz = lib().gtf_pfock_csr(self.h, ptr.ctypes.data_as(_ip), sid.ctypes.data_as(_ip),
+                                  srid.ctypes.data_as(_ip), _p(sval))
+        return dict(nnz=nnz, shellptr=ptr, shellid=sid[:nnz], shellrid=srid[:nnz], shellvalue=sval[:nnz])
+
+    def class_counts(self):
+        c = np.zeros(21, dtype=np.int64)
+        lib().gtf_pfock_class_counts(self.h, c.ctypes.data_as(_lp))
+        return c
+
+    def quartets(self):
+        n = lib().gtf_pfock_quartets(self.h, None, 0)
+        q = np.zeros((max(n, 1), 4), dtype=np.int32)
+        lib().gtf_pfock_quartets(self.h, q.ctypes.data_as(_ip), n)
+        return q[:n]
+
+    def device_ptrs(self, index=0):
+        d, j, k = _vp(), _vp(), _vp()
+        _check(lib().gtf_pfock_device_ptrs(self.h, index, ctypes.byref(d), ctypes.byref(j), ctypes.byref(k)),
+               "gtf_pfock_device_ptrs")
+        return d.value, j.value, k.value
+
+    def stream(self):
+        return lib().gtf_pfock_stream(self.h)
+
+
+class ERD:
+    """CInt ERD handle: shell-quartet ERIs computed by the GPU kernels."""
+
+    def __init__(self, basis, nthreads=1):
+        self.basis = basis
+        self.h = _vp()
+        _check(lib().CInt_createERD(basis.h, ctypes.byref(self.h), nthreads), "CInt_createERD")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().CInt_destroyERD(self.h)
+            self.h = None
+
+    def quartet(self, M, N, P, Q):
+        ptr = _dp()
+        n = ctypes.c_int()
+        _check(lib().CInt_computeShellQuartet(self.basis.h, self.h, 0, M, N, P, Q, ctypes.byref(ptr),
+                                              ctypes.byref(n)), "CInt_computeShellQuartet")
+        return np.ctypeslib.as_array(ptr, shape=(n.value,)).copy()
+
+    def quartets(self, q):
+        q = np.ascontiguousarray(q, dtype=np.int32).reshape(-1, 4)
+        d = np.array(self.basis.shell_dim)
+        sizes = d[q[:, 0]] * d[q[:, 1]] * d[q[:, 2]] * d[q[:, 3]]
+        out = np.zeros(int(sizes.sum()))
+        _check(lib().gtf_erd_quartets(self.h, len(q), q.ctypes.data_as(_ip), _p(out)), "gtf_erd_quartets")
+        offs = np.concatenate([[0], np.cumsum(sizes)])
+        return [out[offs[i]:offs[i + 1]] for i in range(len(q))]
+
+
+def dgemm(A, B, transB=False):
+    """FP64 MFMA GEMM on the GPU: A @ B, or A @ B.T when transB."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    n = A.shape[0]
+    C = np.zeros((n, n))
+    _check(lib().gtf_dgemm_host(n, _p(A), _p(B), _p(C), 1 if transB else 0), "gtf_dgemm_host")
+    return C
+
+
+class Purif:
+    """purif driver (pscf/purif.h:78-86) on one GPU."""
+
+    def __init__(self, basis):
+        self.h = lib().create_purif(basis.h, 1, 1, 1)
+        if not self.h:
+            raise GtfError("create_purif failed")
+        self.n = basis.nbf
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().destroy_purif(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _fields(self):
+        return ctypes.cast(self.h, ctypes.POINTER(PurifT)).contents
+
+    def set_xs(self, X, S):
+        f = self._fields()
+        n = self.n
+        ctypes.memmove(f.X_block, np.ascontiguousarray(X).ctypes.data, 8 * n * n)
+        ctypes.memmove(f.S_block, np.ascontiguousarray(S).ctypes.data, 8 * n * n)
+
+    def diis(self, pfock, D, F, it):
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        F = np.array(F, dtype=np.float64, order="C")
+        lib().compute_diis(pfock.h if pfock is not None else None, self.h, _p(D), _p(F), it)
+        return F
+
+    def purify(self, F):
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        D = np.zeros((self.n, self.n))
+        it = lib().compute_purification(self.h, _p(F), _p(D))
+        if it < 0:
+            raise GtfError("compute_purification failed")
+        return D, it
+
+
+class PurifT(ctypes.Structure):
+    """Field layout of purif_t (include/purif.h)."""
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "runpurif", "rundgemm", "nbf", "nobtls", "np_purif", "nprow_purif", "npcol_purif", "npgrd_purif",
+        "srow_purif", "scol_purif", "nrows_purif", "ncols_purif", "ldx", "meshsize")] + [
+        ("H_block", _dp), ("X_block", _dp), ("S_block", _dp), ("F_block", _dp), ("D_block", _dp),
+        ("len_diis", ctypes.c_int), ("bmax_id", ctypes.c_int), ("bmax", ctypes.c_double),
+        ("b_mat", ctypes.c_double * 121),
+        ("timedgemm", ctypes.c_double), ("timepdgemm", ctypes.c_double), ("timetr", ctypes.c_double),
+        ("timepass", ctypes.c_double), ("dev", _vp)]
+
+
+def scf(gbs, xyz, niters, tolscr=1e-11, verbose=False):
+    """pscf/scf.c:189-474 on one GPU with a zero initial density
+    (core-Hamiltonian guess).  Returns (energies, last D, last F = H + G(D))."""
+    basis = BasisSet(gbs, xyz)
+    pf = PFock(basis, tolscr=tolscr)
+    pu = Purif(basis)
+    S = pf.overlap()
+    X = pf.x_matrix()
+    H = pf.core_hamiltonian()
+    pu.set_xs(X, S)
+    n = basis.nbf
+    D = np.zeros((n, n))
+    enuc = basis.nuc_energy()
+    e0 = -1.0
+    energies = []
+    F = None
+    Dlast = D
+    for it in range(niters):
+        G = pf.fock(D)
+        F = G + H
+        e = float(np.sum(D * (H + F)))
+        energies.append(e + enuc)
+        Dlast = D
+        if verbose:
+            print("  iter %d energy %.10f" % (it, e + enuc), flush=True)
+        if it > 0 and abs(e - e0) < 1e-11:
+            break
+        e0 = e
+        Fo = pu.diis(pf, D, F, it)
+        D, _ = pu.purify(Fo)
+    pu.close()
+    pf.close()
+    return np.array(energies), Dlast, F

@@ -1,0 +1,156 @@
+// eri_md.h -- McMurchie-Davidson electron-repulsion integrals for one shell
+// quartet class, fully unrolled at compile time.
+//
+//   (ab|cd) = 2 pi^(5/2) / (p q sqrt(p+q))
+//             sum_{tuv} E^{ab}_{tuv} sum_{t'u'v'} (-1)^{t'+u'+v'} E^{cd}_{t'u'v'}
+//             R_{t+t',u+u',v+v'}(alpha, P - Q),   alpha = p q / (p + q)
+//
+// (McMurchie and Davidson, J. Comput. Phys. 26, 218 (1978)).  The Hermite
+// coefficients E are stored per primitive pair already contracted with the
+// spherical transform, the contraction coefficients, exp(-mu |AB|^2) and 1/p,
+// so a primitive quartet costs the Boys function, the R recursion and two
+// small matrix products.  A thread owns NC consecutive bra components of one
+// contracted quartet and all ket components.
+#pragma once
+#include "boys.h"
+#include "gtf_internal.h"
+
+#define GTF_TWO_PI_2_5 34.986836655249725693  // 2 pi^(5/2)
+
+// R^{0}_{tuv} for t+u+v <= L, in place (stage n = L .. 0):
+//   R^n_{000} = pref (-2 alpha)^n F_n(T)
+//   R^n_{t+1,u,v} = t R^{n+1}_{t-1,u,v} + X R^{n+1}_{t,u,v}   (same for u/Y, v/Z)
+template <int L, typename Real>
+__device__ __forceinline__ void hermite_R(Real* R, const Real* F, Real pref, Real m2a, Real X, Real Y,
+                                          Real Z)
+{
+    Real pw[L + 1];
+    pw[0] = pref;
+    static_for<1, L + 1>([&](auto n_) {
+        constexpr int n = decltype(n_)::value;
+        pw[n] = pw[n - 1] * m2a;
+    });
+    R[0] = pw[L] * F[L];
+    static_for<0, L>([&](auto s_) {
+        constexpr int n = L - 1 - decltype(s_)::value;
+        constexpr int top = nherm(L - n);
+        static_for<0, top - 1>([&](auto i_) {
+            constexpr int h = top - 1 - decltype(i_)::value;
+            constexpr int t = herm_t(h), u = herm_u(h), v = herm_v(h);
+            if constexpr (t > 0) {
+                Real r = X * R[hidx(t - 1, u, v)];
+                if constexpr (t > 1) r += Real(t - 1) * R[hidx(t - 2, u, v)];
+                R[h] = r;
+            } else if constexpr (u > 0) {
+                Real r = Y * R[hidx(t, u - 1, v)];
+                if constexpr (u > 1) r += Real(u - 1) * R[hidx(t, u - 2, v)];
+                R[h] = r;
+            } else {
+                Real r = Z * R[hidx(t, u, v - 1)];
+                if constexpr (v > 1) r += Real(v - 1) * R[hidx(t, u, v - 2)];
+                R[h] = r;
+            }
+        });
+        R[0] = pw[n] * F[n];
+    });
+}
+
+template <int h1, int h2>
+__host__ __device__ constexpr int hsum()
+{
+    return hidx(herm_t(h1) + herm_t(h2), herm_u(h1) + herm_u(h2), herm_v(h1) + herm_v(h2));
+}
+
+template <int LA, int LB, int LC, int LD>
+struct EriShape {
+    static constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
+    static constexpr int HAB = nherm(LAB), HCD = nherm(LCD), HL = nherm(L);
+    static constexpr int NA = nsph(LA), NB = nsph(LB), NCc = nsph(LC), ND = nsph(LD);
+    static constexpr int NAB = NA * NB, NCD = NCc * ND;
+    static constexpr int ESZ_AB = NAB * HAB, ESZ_CD = NCD * HCD;
+};
+
+// Register budget heuristic: the largest divisor NC of NAB whose live set
+// R + NC*(Ebra + V + acc) stays within ~128 doubles.
+template <int LA, int LB, int LC, int LD>
+__host__ __device__ constexpr int default_nc()
+{
+    using S = EriShape<LA, LB, LC, LD>;
+    int best = 1;
+    for (int nc = 1; nc <= S::NAB; ++nc) {
+        if (S::NAB % nc) continue;
+        int reg = S::HL + nc * (S::HAB + S::HCD + S::NCD) + S::L + 8;
+        if (reg <= 128) best = nc;
+    }
+    return best;
+}
+
+// Contracted integrals (bra components c0..c0+NC-1, all ket components) of
+// the quartet (bra pair | ket pair).  acc[c][cd], cd = c_sph * nsph(LD) + d_sph.
+template <int LA, int LB, int LC, int LD, int NC, typename Real>
+__device__ __forceinline__ void eri_block(const PairView& pv, int bra, int ket, int c0,
+                                          const double* __restrict__ boys_tab,
+                                          Real (&acc)[NC][EriShape<LA, LB, LC, LD>::NCD])
+{
+    using S = EriShape<LA, LB, LC, LD>;
+    constexpr int L = S::L, HAB = S::HAB, HCD = S::HCD, HL = S::HL, NCD = S::NCD;
+    static_for<0, NC>([&](auto c_) {
+        static_for<0, NCD>([&](auto d_) { acc[decltype(c_)::value][decltype(d_)::value] = Real(0); });
+    });
+    const int nb = pv.nprim[bra], nk = pv.nprim[ket];
+    const double4* __restrict__ pb = pv.prim + pv.poff[bra];
+    const double4* __restrict__ pk = pv.prim + pv.poff[ket];
+    const double* __restrict__ Eb = pv.E + pv.eoff[bra] + (long)c0 * HAB;
+    const double* __restrict__ Ek0 = pv.E + pv.eoff[ket];
+    for (int i = 0; i < nb; ++i) {
+        const double4 B = pb[i];
+        Real eb[NC][HAB];
+        static_for<0, NC>([&](auto c_) {
+            constexpr int c = decltype(c_)::value;
+            static_for<0, HAB>([&](auto h_) {
+                constexpr int h = decltype(h_)::value;
+                eb[c][h] = Real(Eb[(long)i * S::ESZ_AB + c * HAB + h]);
+            });
+        });
+        for (int j = 0; j < nk; ++j) {
+            const double4 K = pk[j];
+            const Real p = Real(B.x), q = Real(K.x);
+            const Real X = Real(B.y - K.y), Y = Real(B.z - K.z), Z = Real(B.w - K.w);
+            const Real s = p + q;
+            const Real rs = Real(1) / s;
+            const Real alpha = p * q * rs;
+            const Real T = alpha * (X * X + Y * Y + Z * Z);
+            Real F[L + 1];
+            boys_eval<L, Real>(T, F, boys_tab);
+            const Real pref = Real(GTF_TWO_PI_2_5) * sqrt(rs);
+            Real R[HL];
+            hermite_R<L, Real>(R, F, pref, Real(-2) * alpha, X, Y, Z);
+            Real V[NC][HCD];
+            static_for<0, HCD>([&](auto h2_) {
+                constexpr int h2 = decltype(h2_)::value;
+                constexpr bool neg = (horder(h2) & 1) != 0;
+                static_for<0, NC>([&](auto c_) {
+                    constexpr int c = decltype(c_)::value;
+                    Real v = Real(0);
+                    static_for<0, HAB>([&](auto h1_) {
+                        constexpr int h1 = decltype(h1_)::value;
+                        v += eb[c][h1] * R[hsum<h1, h2>()];
+                    });
+                    V[c][h2] = neg ? -v : v;
+                });
+            });
+            const double* __restrict__ ek = Ek0 + (long)j * S::ESZ_CD;
+            static_for<0, NCD>([&](auto cd_) {
+                constexpr int cd = decltype(cd_)::value;
+                static_for<0, HCD>([&](auto h2_) {
+                    constexpr int h2 = decltype(h2_)::value;
+                    const Real e = Real(ek[cd * HCD + h2]);
+                    static_for<0, NC>([&](auto c_) {
+                        constexpr int c = decltype(c_)::value;
+                        acc[c][cd] += e * V[c][h2];
+                    });
+                });
+            });
+        }
+    }
+}

@@ -1,0 +1,131 @@
+// gtf_internal.h -- shared definitions of the HIP Fock engine (host + device).
+//
+// Angular momentum up to d (L <= 2) per shell, spherical functions.
+// Shell pairs are stored canonically with L(s1) >= L(s2); a pair type is
+//   0 ss, 1 ps, 2 pp, 3 ds, 4 dp, 5 dd
+// and a quartet class (bra type A >= ket type B) is A*(A+1)/2 + B (21 classes).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <type_traits>
+#include <utility>
+
+#define GTF_LMAX 2
+#define GTF_NPAIRTYPES 6
+#define GTF_NCLASSES 21
+
+// Boys-function grid: F_m(T0) for T0 = i * BOYS_DELTA, i = 0..BOYS_NGRID-1,
+// m = 0..BOYS_NCOL-1; T >= BOYS_TMAX uses the asymptotic form.
+#define BOYS_DELTA 0.05
+#define BOYS_INV_DELTA 20.0
+#define BOYS_NGRID 721
+#define BOYS_NCOL 16
+#define BOYS_TMAX 36.0
+#define BOYS_NTAYLOR 7
+
+#define GTF_CHECK(call)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            fprintf(stderr, "gtfock_amd: HIP error %s at %s:%d\n", hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                                   \
+            return -1;                                                                     \
+        }                                                                                  \
+    } while (0)
+
+__host__ __device__ constexpr int nherm(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+__host__ __device__ constexpr int nsph(int l) { return 2 * l + 1; }
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+__host__ __device__ constexpr int pair_type(int la, int lb) { return la * (la + 1) / 2 + lb; }
+__host__ __device__ constexpr int class_index(int A, int B) { return A * (A + 1) / 2 + B; }
+
+// Hermite / cartesian component index: order k = t+u+v, then t descending,
+// then v ascending.  The same ordering serves cartesian components
+// (lx, ly, lz) of one shell when restricted to order k.
+__host__ __device__ constexpr int hidx(int t, int u, int v)
+{
+    return (t + u + v) * (t + u + v + 1) * (t + u + v + 2) / 6 +
+           ((u + v) * (u + v + 1)) / 2 + v;
+}
+__host__ __device__ constexpr int horder(int h)
+{
+    int k = 0;
+    while (nherm(k) <= h) ++k;
+    return k;
+}
+__host__ __device__ constexpr int herm_t(int h)
+{
+    int k = horder(h);
+    int r = h - (k == 0 ? 0 : nherm(k - 1));
+    int s = 0;  // s = u + v
+    while ((s + 1) * (s + 2) / 2 <= r) ++s;
+    return k - s;
+}
+__host__ __device__ constexpr int herm_v(int h)
+{
+    int k = horder(h);
+    int r = h - (k == 0 ? 0 : nherm(k - 1));
+    int s = 0;
+    while ((s + 1) * (s + 2) / 2 <= r) ++s;
+    return r - s * (s + 1) / 2;
+}
+__host__ __device__ constexpr int herm_u(int h) { return horder(h) - herm_t(h) - herm_v(h); }
+
+// compile-time loop: f(std::integral_constant<int, i>) for i in [B, E)
+template <int B, int E, typename F>
+__host__ __device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// Device views of the engine's resident data.
+struct PairView {
+    const int* s1;       // canonical first shell
+    const int* s2;       // canonical second shell
+    const int* nprim;    // primitive pairs
+    const int* poff;     // offset into prim[]
+    const long* eoff;    // offset into E[]
+    const double4* prim; // (p, Px, Py, Pz) per primitive pair
+    const double* E;     // Hermite->spherical coefficients, scaled by c_i c_j K_ab / p
+};
+
+struct BasisView {
+    int nshells, nbf, natoms;
+    const int* L;
+    const int* nprim;
+    const int* poff;
+    const int* fstart;
+    const int* atom;
+    const double* alpha;
+    const double* coef;
+    const double* center;    // 3 per shell (bohr)
+    const double* atom_xyz;  // 3 per atom
+    const double* atom_Z;
+};
+
+// Spherical transform rows (m = -l..l) over cartesian components of order l
+// in the hidx order (xx, xy, xz, yy, yz, zz for l = 2).
+__host__ __device__ inline double sph_coef(int l, int m, int c)
+{
+    if (l == 0) return 1.0;
+    if (l == 1) return m == c ? 1.0 : 0.0;
+    const double r3 = 1.7320508075688772;
+    switch (m) {
+    case 0: return c == 1 ? r3 : 0.0;                              // xy
+    case 1: return c == 4 ? r3 : 0.0;                              // yz
+    case 2: return c == 5 ? 1.0 : ((c == 0 || c == 3) ? -0.5 : 0.0); // z^2 - r^2/2
+    case 3: return c == 2 ? r3 : 0.0;                              // xz
+    case 4: return c == 0 ? 0.5 * r3 : (c == 3 ? -0.5 * r3 : 0.0); // (x^2 - y^2)
+    }
+    return 0.0;
+}
+
+// host-side entry points of the kernel files
+struct EriLaunch;

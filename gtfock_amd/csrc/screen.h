@@ -1,0 +1,18 @@
+// screen.h -- host launchers of screen.hip and linalg.hip.
+#pragma once
+#include "gtf_internal.h"
+
+int gtf_build_pairs(hipStream_t st, const BasisView& b, int npairs, const int* s1, const int* s2,
+                    const int* poff, const long* eoff, double4* prim, double* E);
+int gtf_count_rows(hipStream_t st, int nbra, const double* vbra, int nket, const double* vket, double tol2,
+                   int same, long* counts, long* row_off);
+int gtf_fill_qlist_diag(hipStream_t st, int n, const int* ids, int2* q);
+
+// linalg.hip
+int gtf_fill(hipStream_t st, double* x, long n, double v);
+int gtf_fill_int(hipStream_t st, int* x, long n, int v);
+int gtf_symmetrize(hipStream_t st, double* A, int n, int ld);
+int gtf_add(hipStream_t st, const double* A, const double* B, double* C, long n);
+// C = alpha * op(A) * op(B)^T-convention GEMM on row-major n x n matrices with leading dimension ld
+// transB: 1 -> C = A * B^T (pdgemm3D_2), 0 -> C = A * B
+int gtf_dgemm(hipStream_t st, int n, int ld, const double* A, const double* B, double* C, int transB);
