@@ -108,6 +108,9 @@ def lib():
         "gtf_pfock_device_ptrs": ([_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                    ctypes.POINTER(_vp)], ctypes.c_int),
         "gtf_pfock_stream": ([_vp], _vp),
+        "gtf_pfock_class_work": ([_vp, _lp, _dp], ctypes.c_int),
+        "gtf_pfock_set_timing": ([_vp, ctypes.c_int], ctypes.c_int),
+        "gtf_pfock_class_times": ([_vp, _dp, _lp], ctypes.c_int),
         "gtf_pfock_device_bytes": ([_vp], ctypes.c_double),
         "gtf_erd_quartets": ([_vp, ctypes.c_int, _ip, _dp], ctypes.c_int),
         "gtf_dgemm_host": ([ctypes.c_int, _dp, _dp, _dp, ctypes.c_int], ctypes.c_int),
@@ -266,6 +269,21 @@ class PFock:
         q = np.zeros((max(n, 1), 4), dtype=np.int32)
         lib().gtf_pfock_quartets(self.h, q.ctypes.data_as(_ip), n)
         return q[:n]
+
+    def class_work(self):
+        nq = np.zeros(21, dtype=np.int64)
+        npq = np.zeros(21)
+        lib().gtf_pfock_class_work(self.h, nq.ctypes.data_as(_lp), _p(npq))
+        return nq, npq
+
+    def set_timing(self, on=True):
+        _check(lib().gtf_pfock_set_timing(self.h, 1 if on else 0), "gtf_pfock_set_timing")
+
+    def class_times(self):
+        ms = np.zeros(21)
+        n = np.zeros(21, dtype=np.int64)
+        lib().gtf_pfock_class_times(self.h, _p(ms), n.ctypes.data_as(_lp))
+        return ms, n
 
     def device_ptrs(self, index=0):
         d, j, k = _vp(), _vp(), _vp()

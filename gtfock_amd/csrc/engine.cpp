@@ -198,8 +198,16 @@ int GtfSystem::screen(double tolscr)
         if (nsig[A] == 0 || nsig[B] == 0) continue;
         if (gtf_count_rows(stream, nsig[A], d_sigv[A], nsig[B], d_sigv[B], tol2, A == B, d_counts, d_rowoff[cls]))
             return -1;
-        SYS_CHECK(hipMemcpyAsync(&nq[cls], d_rowoff[cls] + nsig[A], sizeof(long), hipMemcpyDeviceToHost, stream));
+        std::vector<long> off(nsig[A] + 1);
+        SYS_CHECK(hipMemcpyAsync(off.data(), d_rowoff[cls], sizeof(long) * off.size(), hipMemcpyDeviceToHost, stream));
         SYS_CHECK(hipStreamSynchronize(stream));
+        nq[cls] = off[nsig[A]];
+        // primitive-quartet work of the class, for the FLOP accounting
+        std::vector<double> kp(nsig[B] + 1, 0.0);
+        for (int j = 0; j < nsig[B]; j++) kp[j + 1] = kp[j] + h_pnprim[h_sig[B][j]];
+        double w = 0.0;
+        for (int r = 0; r < nsig[A]; r++) w += (double)h_pnprim[h_sig[A][r]] * kp[off[r + 1] - off[r]];
+        nprimq[cls] = w;
     }
     screened = true;
     return 0;
@@ -252,7 +260,23 @@ int GtfSystem::jk(const double* D, double* J, double* K, int ld)
         a.K = K;
         a.ld = ld;
         const int blocks = std::min(a.ntasks, 2048);
+        if (timing) SYS_CHECK(hipEventRecord(tev[2 * cls], stream));
         if (launch(ci->jk, a, blocks, stream)) return -1;
+        if (timing) SYS_CHECK(hipEventRecord(tev[2 * cls + 1], stream));
+    }
+    return 0;
+}
+
+int GtfSystem::collect_times()
+{
+    if (!timing) return 0;
+    for (int cls = 0; cls < GTF_NCLASSES; cls++) {
+        if (nq[cls] == 0) continue;
+        SYS_CHECK(hipEventSynchronize(tev[2 * cls + 1]));
+        float ms = 0.f;
+        SYS_CHECK(hipEventElapsedTime(&ms, tev[2 * cls], tev[2 * cls + 1]));
+        class_ms[cls] += ms;
+        class_launches[cls] += 1;
     }
     return 0;
 }
@@ -273,6 +297,8 @@ int GtfSystem::store(int cls, int nq_, const int2* d_qlist, double* d_out)
 
 void GtfSystem::release()
 {
+    for (hipEvent_t& e : tev)
+        if (e) { (void)hipEventDestroy(e); e = nullptr; }
     for (void* p : allocs) (void)hipFree(p);
     allocs.clear();
     dev_bytes = 0;

@@ -39,6 +39,13 @@ struct GtfSystem {
     long* d_rowoff[GTF_NCLASSES] = {nullptr};
     int* d_counters = nullptr;
     size_t dev_bytes = 0;
+    // primitive quartets per class (sum over quartets of nprim(bra) * nprim(ket))
+    double nprimq[GTF_NCLASSES] = {0};
+    // optional per-class kernel timing with events on the launch stream
+    bool timing = false;
+    hipEvent_t tev[2 * GTF_NCLASSES] = {nullptr};
+    double class_ms[GTF_NCLASSES] = {0};
+    long class_launches[GTF_NCLASSES] = {0};
 
     int init(const BasisSet* b, hipStream_t st);
     // Schwarz screening (pfock/screening.c:15-180) and quartet compaction
@@ -50,6 +57,8 @@ struct GtfSystem {
     PairView pair_view() const;
     // fused ERI + J/K digestion over all screened quartets of every class
     int jk(const double* D, double* J, double* K, int ld);
+    // accumulate the per-class event times of the last jk() into class_ms
+    int collect_times();
     // contracted integrals of a list of (bra, ket) canonical pair quartets of one class
     int store(int cls, int nq_, const int2* d_qlist, double* d_out);
     void release();

@@ -337,6 +337,7 @@ PFockStatus_t PFock_computeFock(BasisSet_t basis, PFock_t pf)
         if (gtf_fill(pf->stream, pf->d_J[i], nn, 0.0) || gtf_fill(pf->stream, pf->d_K[i], nn, 0.0))
             return PFOCK_STATUS_EXECUTION_FAILED;
         if (pf->sys.jk(pf->d_D[i], pf->d_J[i], pf->d_K[i], pf->nbf)) return PFOCK_STATUS_EXECUTION_FAILED;
+        if (pf->sys.collect_times()) return PFOCK_STATUS_EXECUTION_FAILED;
         if (gtf_symmetrize(pf->stream, pf->d_J[i], pf->nbf, pf->nbf) ||
             gtf_symmetrize(pf->stream, pf->d_K[i], pf->nbf, pf->nbf))
             return PFOCK_STATUS_EXECUTION_FAILED;
@@ -569,6 +570,38 @@ int gtf_pfock_device_ptrs(PFock_t pf, int index, double** D, double** J, double*
 }
 
 void* gtf_pfock_stream(PFock_t pf) { return (void*)pf->stream; }
+
+int gtf_pfock_class_work(PFock_t pf, long* nquartets, double* nprimquartets)
+{
+    for (int c = 0; c < GTF_NCLASSES; c++) {
+        nquartets[c] = pf->sys.nq[c];
+        nprimquartets[c] = pf->sys.nprimq[c];
+    }
+    return GTF_NCLASSES;
+}
+
+int gtf_pfock_set_timing(PFock_t pf, int on)
+{
+    GtfSystem& s = pf->sys;
+    if (on && !s.timing)
+        for (hipEvent_t& e : s.tev)
+            if (!e && hipEventCreate(&e) != hipSuccess) return -1;
+    s.timing = on != 0;
+    for (int c = 0; c < GTF_NCLASSES; c++) {
+        s.class_ms[c] = 0.0;
+        s.class_launches[c] = 0;
+    }
+    return 0;
+}
+
+int gtf_pfock_class_times(PFock_t pf, double* ms, long* launches)
+{
+    for (int c = 0; c < GTF_NCLASSES; c++) {
+        ms[c] = pf->sys.class_ms[c];
+        launches[c] = pf->sys.class_launches[c];
+    }
+    return GTF_NCLASSES;
+}
 
 double gtf_pfock_device_bytes(PFock_t pf) { return (double)pf->sys.dev_bytes; }
 

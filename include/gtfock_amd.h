@@ -37,6 +37,16 @@ long gtf_pfock_quartets(PFock_t pfock, int *quartets, long cap);
  * K[index] (-K accumulator), row-major with leading dimension nbf. */
 int gtf_pfock_device_ptrs(PFock_t pfock, int index, double **D, double **J, double **K);
 
+/* Per quartet class: screened quartets and primitive quartets (sum of
+ * nprim(bra) * nprim(ket)), the work units of the FLOP accounting. */
+int gtf_pfock_class_work(PFock_t pfock, long *nquartets, double *nprimquartets);
+
+/* Per-class kernel timing with HIP events recorded on the engine's stream
+ * around each class launch; enabling resets the accumulators. */
+int gtf_pfock_set_timing(PFock_t pfock, int on);
+/* Accumulated milliseconds and launch counts per class since enabling. */
+int gtf_pfock_class_times(PFock_t pfock, double *ms, long *launches);
+
 /* The engine's HIP stream (hipStream_t). */
 void *gtf_pfock_stream(PFock_t pfock);
 
