@@ -12,13 +12,16 @@
 #pragma once
 #include "gtf_internal.h"
 
-template <int M, typename Real>
+// tab: the grid with row stride STRIDE; STRIDE == BOYS_NCOL is the full
+// table (columns m = 0..15), STRIDE == BOYS_NTAYLOR the 7 columns M..M+6
+// staged in LDS for one class.
+template <int M, typename Real, int STRIDE = BOYS_NCOL>
 __device__ __forceinline__ void boys_eval(Real T, Real* F, const double* __restrict__ tab)
 {
     if (T < Real(BOYS_TMAX)) {
         int i = (int)(T * Real(BOYS_INV_DELTA) + Real(0.5));
         Real dt = Real(i) * Real(BOYS_DELTA) - T;
-        const double* row = tab + i * BOYS_NCOL + M;
+        const double* row = tab + i * STRIDE + (STRIDE == BOYS_NCOL ? M : 0);
         Real f = Real(row[6]);
         f = Real(row[5]) + dt * (Real(1.0 / 6.0)) * f;
         f = Real(row[4]) + dt * (Real(1.0 / 5.0)) * f;

@@ -107,8 +107,12 @@ int GtfSystem::init(const BasisSet* b, hipStream_t st)
         return -1;
     BasisView bv{nshells, nbf, natoms, d_L, d_nprim, d_poff, d_fstart, d_atom,
                  d_alpha, d_coef, d_center, d_atom_xyz, d_atom_Z};
-    if (gtf_build_pairs(st, bv, npairs, d_s1, d_s2, d_ppoff, d_peoff, d_prim, d_E)) return -1;
+    if (gtf_build_pairs(st, bv, npairs, d_s1, d_s2, d_ppoff, d_peoff, d_prim, d_E, d_pnprim, prim_eps)) return -1;
+    // surviving primitive pairs per shell pair
+    SYS_CHECK(hipMemcpyAsync(h_pnprim.data(), d_pnprim, sizeof(int) * npairs, hipMemcpyDeviceToHost, st));
     SYS_CHECK(hipStreamSynchronize(st));
+    nprimpairs_kept = 0;
+    for (int id = 0; id < npairs; id++) nprimpairs_kept += h_pnprim[id];
     return 0;
 }
 
@@ -117,15 +121,22 @@ PairView GtfSystem::pair_view() const
     return PairView{d_s1, d_s2, d_pnprim, d_ppoff, d_peoff, d_prim, d_E};
 }
 
-static int launch(eri_kernel_t fn, EriArgs& a, int blocks, hipStream_t st)
+static int launch(eri_kernel_t fn, EriArgs& a, int blocks, hipStream_t st, size_t lds = 0)
 {
     void* args[] = {&a};
-    SYS_CHECK(hipLaunchKernel((const void*)fn, dim3(blocks), dim3(256), args, 0, st));
+    SYS_CHECK(hipLaunchKernel((const void*)fn, dim3(blocks), dim3(256), args, lds, st));
     return 0;
 }
 
 int GtfSystem::screen(double tolscr)
 {
+    if (const char* s = getenv("GTF_JK_TILES")) use_tiles = atoi(s) != 0;
+    {
+        int dev = 0, lds = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+            max_lds = (size_t)lds;
+    }
     tol = tolscr;
     tol2 = tolscr * tolscr;
     const PairView pv = pair_view();
@@ -208,6 +219,23 @@ int GtfSystem::screen(double tolscr)
         double w = 0.0;
         for (int r = 0; r < nsig[A]; r++) w += (double)h_pnprim[h_sig[A][r]] * kp[off[r + 1] - off[r]];
         nprimq[cls] = w;
+        // bra-row tiles: up to GTF_TILE_UNITS work units of one row each
+        const EriClassInfo* ci = gtf_eri_class(cls);
+        const int NA = 2 * ci->la + 1, NB = 2 * ci->lb + 1;
+        tile_lds[cls] = sizeof(double) * ((size_t)(NA + NB) * nbf + NA * NB);
+        if (use_tiles && tile_lds[cls] <= max_lds) {
+            const int tq = std::max(1, GTF_TILE_UNITS / ci->nsplit);
+            std::vector<int4> tiles;
+            for (int r = 0; r < nsig[A]; r++) {
+                const long cnt = off[r + 1] - off[r];
+                for (long j0 = 0; j0 < cnt; j0 += tq)
+                    tiles.push_back(make_int4(r, (int)j0, (int)std::min<long>(cnt, j0 + tq), 0));
+            }
+            ntiles[cls] = (int)tiles.size();
+            d_tiles[cls] = dalloc<int4>(tiles.size());
+            if (!d_tiles[cls]) return -2;
+            if (upload(d_tiles[cls], tiles, stream)) return -1;
+        }
     }
     screened = true;
     return 0;
@@ -259,9 +287,15 @@ int GtfSystem::jk(const double* D, double* J, double* K, int ld)
         a.J = J;
         a.K = K;
         a.ld = ld;
-        const int blocks = std::min(a.ntasks, 2048);
         if (timing) SYS_CHECK(hipEventRecord(tev[2 * cls], stream));
-        if (launch(ci->jk, a, blocks, stream)) return -1;
+        if (ntiles[cls] > 0) {
+            a.tiles = d_tiles[cls];
+            a.ntiles = ntiles[cls];
+            a.nbf = nbf;
+            if (launch(ci->jk_tile, a, std::min(a.ntiles, 2048), stream, tile_lds[cls])) return -1;
+        } else if (launch(ci->jk, a, std::min(a.ntasks, 2048), stream)) {
+            return -1;
+        }
         if (timing) SYS_CHECK(hipEventRecord(tev[2 * cls + 1], stream));
     }
     return 0;

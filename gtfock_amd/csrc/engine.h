@@ -23,7 +23,9 @@ struct GtfSystem {
     long* d_peoff = nullptr;
     double4* d_prim = nullptr;
     double* d_E = nullptr;
-    long nprimpairs = 0, nE = 0;
+    long nprimpairs = 0, nE = 0, nprimpairs_kept = 0;
+    // primitive-pair screening threshold on max |E| of a pair block
+    double prim_eps = 1e-20;
     // Schwarz values per pair and screening results
     std::vector<double> h_v;
     double vmax = 0.0;
@@ -39,6 +41,12 @@ struct GtfSystem {
     long* d_rowoff[GTF_NCLASSES] = {nullptr};
     int* d_counters = nullptr;
     size_t dev_bytes = 0;
+    // bra-row tiles per class for the LDS-strip kernel (empty: global-atomic kernel)
+    int ntiles[GTF_NCLASSES] = {0};
+    int4* d_tiles[GTF_NCLASSES] = {nullptr};
+    size_t tile_lds[GTF_NCLASSES] = {0};
+    size_t max_lds = 65536;
+    bool use_tiles = true;
     // primitive quartets per class (sum over quartets of nprim(bra) * nprim(ket))
     double nprimq[GTF_NCLASSES] = {0};
     // optional per-class kernel timing with events on the launch stream

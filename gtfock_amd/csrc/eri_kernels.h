@@ -13,6 +13,10 @@
 #pragma once
 #include "eri_md.h"
 
+#ifndef GTF_ATOMIC_ADD
+#define GTF_ATOMIC_ADD(p, v) atomicAdd((p), (v))
+#endif
+
 enum { EPI_JK = 0, EPI_DIAG = 1, EPI_STORE = 2 };
 
 struct EriArgs {
@@ -37,6 +41,10 @@ struct EriArgs {
     unsigned long long* vbits;
     // EPI_STORE
     double* out;
+    // bra-row tiles of the LDS-strip J/K kernel: (bra row, first ket, end ket)
+    const int4* tiles;
+    int ntiles;
+    int nbf;
 };
 
 __device__ __forceinline__ int find_row(const long* __restrict__ off, int nrows, long q)
@@ -50,8 +58,18 @@ __device__ __forceinline__ int find_row(const long* __restrict__ off, int nrows,
     return lo;
 }
 
-template <int LA, int LB, int LC, int LD, int NC, int EPI>
-__device__ __forceinline__ void eri_unit(const EriArgs& a, long u)
+template <int LA, int LB, int LC, int LD, int NC, bool STRIP>
+__device__ __forceinline__ void digest(const EriArgs& a, int bra, int ket, int c0,
+                                       const double (&acc)[NC][EriShape<LA, LB, LC, LD>::NCD], double* jmn,
+                                       double* strip, int nbf, int unused);
+
+// the Boys-grid columns L..L+6 of one class staged in LDS (40 KB)
+#ifndef GTF_BOYS_LDS
+#define GTF_BOYS_LDS 0
+#endif
+
+template <int LA, int LB, int LC, int LD, int NC, int EPI, int BSTRIDE>
+__device__ __forceinline__ void eri_unit(const EriArgs& a, long u, const double* __restrict__ boys)
 {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr int NSPLIT = S::NAB / NC;
@@ -69,7 +87,7 @@ __device__ __forceinline__ void eri_unit(const EriArgs& a, long u)
         ket = a.ket_list[q - a.row_off[r]];
     }
     double acc[NC][NCD];
-    eri_block<LA, LB, LC, LD, NC, double>(a.pv, bra, ket, c0, a.boys, acc);
+    eri_block<LA, LB, LC, LD, NC, double, BSTRIDE>(a.pv, bra, ket, c0, boys, acc);
 
     if constexpr (EPI == EPI_STORE) {
         double* o = a.out + q * (long)(S::NAB * NCD) + (long)c0 * NCD;
@@ -91,79 +109,109 @@ __device__ __forceinline__ void eri_unit(const EriArgs& a, long u)
         });
         atomicMax(a.vbits + bra, (unsigned long long)__double_as_longlong(mx));
     } else {
-        const int M = a.pv.s1[bra], N = a.pv.s2[bra], P = a.pv.s1[ket], Q = a.pv.s2[ket];
-        const int f1 = (M != N), f2 = (P != Q), f3 = (bra != ket);
-        const double cJ1 = 2.0 * (1 + f1) * (1 + f2);
-        const double cJ2 = 2.0 * f3 * (1 + f1) * (1 + f2);
-        const double cMP = (1 + f3), cNP = f1 * (1 + f3), cMQ = f2 * (1 + f3), cNQ = f1 * f2 * (1 + f3);
-        const int ld = a.ld;
-        const int oM = a.fstart[M], oN = a.fstart[N], oP = a.fstart[P], oQ = a.fstart[Q];
-        const double* __restrict__ D = a.D;
-        double* J = a.J;
-        double* K = a.K;
-        int mrow[NC], ncol[NC];
-        double dmn[NC];
-        static_for<0, NC>([&](auto c_) {
-            constexpr int c = decltype(c_)::value;
-            const int ab = c0 + c;
-            mrow[c] = oM + ab / NB;
-            ncol[c] = oN + ab % NB;
-            dmn[c] = D[(long)mrow[c] * ld + ncol[c]];
+        digest<LA, LB, LC, LD, NC, false>(a, bra, ket, c0, acc, nullptr, nullptr, 0, 0);
+    }
+}
+
+// update_F digestion of one thread's NC bra components of quartet (bra|ket).
+// STRIP = false: FP64 atomics straight into J and K.
+// STRIP = true:  the bra-side blocks (J_MN and the exchange blocks K_MP, K_NP,
+// K_MQ, K_NQ, which all lie in the rows of shells M and N) go to LDS
+// (jmn[NA*NB], strip[(NA+NB) * nbf], row r < NA is function fM + r, row
+// NA + r is fN + r); only J_PQ is added to HBM.
+template <int LA, int LB, int LC, int LD, int NC, bool STRIP>
+__device__ __forceinline__ void digest(const EriArgs& a, int bra, int ket, int c0,
+                                       const double (&acc)[NC][EriShape<LA, LB, LC, LD>::NCD], double* jmn,
+                                       double* strip, int nbf, int unused)
+{
+    (void)unused;
+    using S = EriShape<LA, LB, LC, LD>;
+    constexpr int NCD = S::NCD, ND = S::ND, NB = S::NB, NA = S::NA;
+    const int M = a.pv.s1[bra], N = a.pv.s2[bra], P = a.pv.s1[ket], Q = a.pv.s2[ket];
+    const int f1 = (M != N), f2 = (P != Q), f3 = (bra != ket);
+    const double cJ1 = 2.0 * (1 + f1) * (1 + f2);
+    const double cJ2 = 2.0 * f3 * (1 + f1) * (1 + f2);
+    const double cMP = (1 + f3), cNP = f1 * (1 + f3), cMQ = f2 * (1 + f3), cNQ = f1 * f2 * (1 + f3);
+    const int ld = a.ld;
+    const int oM = a.fstart[M], oN = a.fstart[N], oP = a.fstart[P], oQ = a.fstart[Q];
+    const double* __restrict__ D = a.D;
+    double* J = a.J;
+    double* K = a.K;
+    int mloc[NC], nloc[NC];
+    double dmn[NC];
+    static_for<0, NC>([&](auto c_) {
+        constexpr int c = decltype(c_)::value;
+        const int ab = c0 + c;
+        mloc[c] = ab / NB;
+        nloc[c] = ab % NB;
+        dmn[c] = D[(long)(oM + mloc[c]) * ld + oN + nloc[c]];
+    });
+    // J_MN += cJ1 * sum_pq D_pq I   and   J_PQ += cJ2 * sum_mn D_mn I
+    static_for<0, NC>([&](auto c_) {
+        constexpr int c = decltype(c_)::value;
+        double s = 0.0;
+        static_for<0, NCD>([&](auto d_) {
+            constexpr int cd = decltype(d_)::value;
+            s += D[(long)(oP + cd / ND) * ld + oQ + cd % ND] * acc[c][cd];
         });
-        // J_MN += cJ1 * sum_pq D_pq I   and   J_PQ += cJ2 * sum_mn D_mn I
-        static_for<0, NC>([&](auto c_) {
-            constexpr int c = decltype(c_)::value;
+        if constexpr (STRIP) atomicAdd(&jmn[mloc[c] * NB + nloc[c]], cJ1 * s);
+        else GTF_ATOMIC_ADD(&J[(long)(oM + mloc[c]) * ld + oN + nloc[c]], cJ1 * s);
+    });
+    if (f3) {
+        static_for<0, NCD>([&](auto d_) {
+            constexpr int cd = decltype(d_)::value;
             double s = 0.0;
-            static_for<0, NCD>([&](auto d_) {
-                constexpr int cd = decltype(d_)::value;
-                s += D[(long)(oP + cd / ND) * ld + oQ + cd % ND] * acc[c][cd];
+            static_for<0, NC>([&](auto c_) {
+                constexpr int c = decltype(c_)::value;
+                s += dmn[c] * acc[c][cd];
             });
-            atomicAdd(&J[(long)mrow[c] * ld + ncol[c]], cJ1 * s);
-        });
-        if (f3) {
-            static_for<0, NCD>([&](auto d_) {
-                constexpr int cd = decltype(d_)::value;
-                double s = 0.0;
-                static_for<0, NC>([&](auto c_) {
-                    constexpr int c = decltype(c_)::value;
-                    s += dmn[c] * acc[c][cd];
-                });
-                atomicAdd(&J[(long)(oP + cd / ND) * ld + oQ + cd % ND], cJ2 * s);
-            });
-        }
-        // K_MP -= cMP sum_q D_nq I,  K_NP -= cNP sum_q D_mq I
-        // K_MQ -= cMQ sum_p D_np I,  K_NQ -= cNQ sum_p D_mp I
-        static_for<0, NC>([&](auto c_) {
-            constexpr int c = decltype(c_)::value;
-            const long rm = (long)mrow[c] * ld, rn = (long)ncol[c] * ld;
-            static_for<0, S::NCc>([&](auto p_) {
-                constexpr int p = decltype(p_)::value;
-                double smp = 0.0, snp = 0.0;
-                static_for<0, ND>([&](auto q_) {
-                    constexpr int qq = decltype(q_)::value;
-                    const double x = acc[c][p * ND + qq];
-                    smp += D[rn + oQ + qq] * x;
-                    snp += D[rm + oQ + qq] * x;
-                });
-                atomicAdd(&K[rm + oP + p], -cMP * smp);
-                if (f1) atomicAdd(&K[rn + oP + p], -cNP * snp);
-            });
-            if (f2) {
-                static_for<0, ND>([&](auto q_) {
-                    constexpr int qq = decltype(q_)::value;
-                    double smq = 0.0, snq = 0.0;
-                    static_for<0, S::NCc>([&](auto p_) {
-                        constexpr int p = decltype(p_)::value;
-                        const double x = acc[c][p * ND + qq];
-                        smq += D[rn + oP + p] * x;
-                        snq += D[rm + oP + p] * x;
-                    });
-                    atomicAdd(&K[rm + oQ + qq], -cMQ * smq);
-                    if (f1) atomicAdd(&K[rn + oQ + qq], -cNQ * snq);
-                });
-            }
+            GTF_ATOMIC_ADD(&J[(long)(oP + cd / ND) * ld + oQ + cd % ND], cJ2 * s);
         });
     }
+    // K_MP -= cMP sum_q D_nq I,  K_NP -= cNP sum_q D_mq I
+    // K_MQ -= cMQ sum_p D_np I,  K_NQ -= cNQ sum_p D_mp I
+    static_for<0, NC>([&](auto c_) {
+        constexpr int c = decltype(c_)::value;
+        const long rm = (long)(oM + mloc[c]) * ld, rn = (long)(oN + nloc[c]) * ld;
+        double* km = STRIP ? strip + (long)mloc[c] * nbf : K + rm;
+        double* kn = STRIP ? strip + (long)(NA + nloc[c]) * nbf : K + rn;
+        static_for<0, S::NCc>([&](auto p_) {
+            constexpr int p = decltype(p_)::value;
+            double smp = 0.0, snp = 0.0;
+            static_for<0, ND>([&](auto q_) {
+                constexpr int qq = decltype(q_)::value;
+                const double x = acc[c][p * ND + qq];
+                smp += D[rn + oQ + qq] * x;
+                snp += D[rm + oQ + qq] * x;
+            });
+            if constexpr (STRIP) {
+                atomicAdd(&km[oP + p], -cMP * smp);
+                if (f1) atomicAdd(&kn[oP + p], -cNP * snp);
+            } else {
+                GTF_ATOMIC_ADD(&km[oP + p], -cMP * smp);
+                if (f1) GTF_ATOMIC_ADD(&kn[oP + p], -cNP * snp);
+            }
+        });
+        if (f2) {
+            static_for<0, ND>([&](auto q_) {
+                constexpr int qq = decltype(q_)::value;
+                double smq = 0.0, snq = 0.0;
+                static_for<0, S::NCc>([&](auto p_) {
+                    constexpr int p = decltype(p_)::value;
+                    const double x = acc[c][p * ND + qq];
+                    smq += D[rn + oP + p] * x;
+                    snq += D[rm + oP + p] * x;
+                });
+                if constexpr (STRIP) {
+                    atomicAdd(&km[oQ + qq], -cMQ * smq);
+                    if (f1) atomicAdd(&kn[oQ + qq], -cNQ * snq);
+                } else {
+                    GTF_ATOMIC_ADD(&km[oQ + qq], -cMQ * smq);
+                    if (f1) GTF_ATOMIC_ADD(&kn[oQ + qq], -cNQ * snq);
+                }
+            });
+        }
+    });
 }
 
 template <int LA, int LB, int LC, int LD, int NC, int EPI>
@@ -171,6 +219,20 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
 {
     if (a.counter) {
         __shared__ int s_task;
+#if GTF_BOYS_LDS
+        constexpr int L = LA + LB + LC + LD;
+        __shared__ double s_boys[BOYS_NGRID * BOYS_NTAYLOR];
+        for (int k = threadIdx.x; k < BOYS_NGRID * BOYS_NTAYLOR; k += 256) {
+            const int row = k / BOYS_NTAYLOR, col = k - row * BOYS_NTAYLOR;
+            s_boys[k] = a.boys[row * BOYS_NCOL + L + col];
+        }
+        __syncthreads();
+        const double* boys = s_boys;
+        constexpr int BS = BOYS_NTAYLOR;
+#else
+        const double* boys = a.boys;
+        constexpr int BS = BOYS_NCOL;
+#endif
         for (;;) {
             if (threadIdx.x == 0) s_task = atomicAdd(a.counter, 1);
             __syncthreads();
@@ -178,15 +240,71 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
             __syncthreads();
             if (task >= a.ntasks) break;
             const long u = (long)task * 256 + threadIdx.x;
-            if (u < a.nunits) eri_unit<LA, LB, LC, LD, NC, EPI>(a, u);
+            if (u < a.nunits) eri_unit<LA, LB, LC, LD, NC, EPI, BS>(a, u, boys);
         }
     } else {
         const long u = (long)blockIdx.x * 256 + threadIdx.x;
-        if (u < a.nunits) eri_unit<LA, LB, LC, LD, NC, EPI>(a, u);
+        if (u < a.nunits) eri_unit<LA, LB, LC, LD, NC, EPI, BOYS_NCOL>(a, u, a.boys);
     }
 }
 
-// launch helpers (eri_launch.hip)
+// Fused ERI + J/K digestion over bra-row tiles.  A tile is up to
+// GTF_TILE_UNITS work units of one bra row; the workgroup accumulates J_MN and
+// the exchange blocks of the bra shells in LDS and flushes them with one HBM
+// atomic per touched element when the tile is done.  Dynamic LDS:
+// (NA + NB) * nbf + NA * NB doubles.
+#define GTF_TILE_UNITS 1024
+
+template <int LA, int LB, int LC, int LD, int NC>
+__global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
+{
+    using S = EriShape<LA, LB, LC, LD>;
+    constexpr int NSPLIT = S::NAB / NC;
+    constexpr int NA = S::NA, NB = S::NB, NCD = S::NCD;
+    extern __shared__ double smem[];
+    __shared__ int s_task;
+    const int nbf = a.nbf;
+    const int nstrip = (NA + NB) * nbf;
+    double* strip = smem;
+    double* jmn = smem + nstrip;
+    for (;;) {
+        if (threadIdx.x == 0) s_task = atomicAdd(a.counter, 1);
+        __syncthreads();
+        const int task = s_task;
+        if (task >= a.ntiles) break;
+        const int4 t = a.tiles[task];
+        const int bra = a.bra_list[t.x];
+        for (int k = threadIdx.x; k < nstrip + NA * NB; k += 256) smem[k] = 0.0;
+        __syncthreads();
+        const int nunits = (t.z - t.y) * NSPLIT;
+        for (int u = threadIdx.x; u < nunits; u += 256) {
+            const int q = u / NSPLIT;
+            const int c0 = (u - q * NSPLIT) * NC;
+            const int ket = a.ket_list[t.y + q];
+            double acc[NC][NCD];
+            eri_block<LA, LB, LC, LD, NC, double, BOYS_NCOL>(a.pv, bra, ket, c0, a.boys, acc);
+            digest<LA, LB, LC, LD, NC, true>(a, bra, ket, c0, acc, jmn, strip, nbf, 0);
+        }
+        __syncthreads();
+        const int M = a.pv.s1[bra], N = a.pv.s2[bra];
+        const int oM = a.fstart[M], oN = a.fstart[N];
+        for (int k = threadIdx.x; k < nstrip; k += 256) {
+            const double v = strip[k];
+            if (v != 0.0) {
+                const int r = k / nbf, col = k - r * nbf;
+                const int row = r < NA ? oM + r : oN + (r - NA);
+                GTF_ATOMIC_ADD(&a.K[(long)row * a.ld + col], v);
+            }
+        }
+        for (int k = threadIdx.x; k < NA * NB; k += 256) {
+            const double v = jmn[k];
+            if (v != 0.0) GTF_ATOMIC_ADD(&a.J[(long)(oM + k / NB) * a.ld + oN + k % NB], v);
+        }
+        __syncthreads();
+    }
+}
+
+// launch helpers (eri_table.cpp)
 typedef void (*eri_kernel_t)(EriArgs);
 struct EriClassInfo {
     int la, lb, lc, ld;
@@ -194,5 +312,6 @@ struct EriClassInfo {
     int nsplit;     // threads per quartet
     int nints;      // integrals per quartet
     eri_kernel_t jk, diag, store;
+    eri_kernel_t jk_tile;
 };
 const EriClassInfo* gtf_eri_class(int cls);

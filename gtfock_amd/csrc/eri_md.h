@@ -87,7 +87,7 @@ __host__ __device__ constexpr int default_nc()
 
 // Contracted integrals (bra components c0..c0+NC-1, all ket components) of
 // the quartet (bra pair | ket pair).  acc[c][cd], cd = c_sph * nsph(LD) + d_sph.
-template <int LA, int LB, int LC, int LD, int NC, typename Real>
+template <int LA, int LB, int LC, int LD, int NC, typename Real, int BSTRIDE = BOYS_NCOL>
 __device__ __forceinline__ void eri_block(const PairView& pv, int bra, int ket, int c0,
                                           const double* __restrict__ boys_tab,
                                           Real (&acc)[NC][EriShape<LA, LB, LC, LD>::NCD])
@@ -121,7 +121,7 @@ __device__ __forceinline__ void eri_block(const PairView& pv, int bra, int ket, 
             const Real alpha = p * q * rs;
             const Real T = alpha * (X * X + Y * Y + Z * Z);
             Real F[L + 1];
-            boys_eval<L, Real>(T, F, boys_tab);
+            boys_eval<L, Real, BSTRIDE>(T, F, boys_tab);
             const Real pref = Real(GTF_TWO_PI_2_5) * sqrt(rs);
             Real R[HL];
             hermite_R<L, Real>(R, F, pref, Real(-2) * alpha, X, Y, Z);

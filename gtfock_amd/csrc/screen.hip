@@ -11,10 +11,14 @@
 #include "gtf_internal.h"
 #include "screen.h"
 
+// Primitive pairs whose largest scaled coefficient |E| is below prim_eps are
+// dropped (their quartets contribute < 1e-15 to any Fock element); the
+// survivors are packed at the front of the pair's slots and nprim_out
+// receives their count.
 __global__ void build_pairs_kernel(BasisView b, int npairs, const int* __restrict__ s1,
                                    const int* __restrict__ s2, const int* __restrict__ poff,
                                    const long* __restrict__ eoff, double4* __restrict__ prim,
-                                   double* __restrict__ E)
+                                   double* __restrict__ E, int* __restrict__ nprim_out, double prim_eps)
 {
     const int pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= npairs) return;
@@ -27,7 +31,7 @@ __global__ void build_pairs_kernel(BasisView b, int npairs, const int* __restric
     for (int d = 0; d < 3; d++) AB2 += (CA[d] - CB[d]) * (CA[d] - CB[d]);
     int k = 0;
     for (int i = 0; i < b.nprim[A]; i++)
-        for (int j = 0; j < b.nprim[B]; j++, k++) {
+        for (int j = 0; j < b.nprim[B]; j++) {
             const double a = b.alpha[b.poff[A] + i], bb = b.alpha[b.poff[B] + j];
             const double p = a + bb, mu = a * bb / p;
             double P[3];
@@ -54,8 +58,8 @@ __global__ void build_pairs_kernel(BasisView b, int npairs, const int* __restric
                         }
             }
             const double scale = b.coef[b.poff[A] + i] * b.coef[b.poff[B] + j] * exp(-mu * AB2) / p;
-            prim[poff[pid] + k] = make_double4(p, P[0], P[1], P[2]);
             double* out = E + eoff[pid] + (long)k * esz;
+            double emax = 0.0;
             const int ca0 = la == 0 ? 0 : nherm(la - 1), cb0 = lb == 0 ? 0 : nherm(lb - 1);
             for (int as = 0; as < na; as++)
                 for (int bs = 0; bs < nb; bs++)
@@ -75,8 +79,14 @@ __global__ void build_pairs_kernel(BasisView b, int npairs, const int* __restric
                             }
                         }
                         out[(as * nb + bs) * H + h] = scale * s;
+                        emax = fmax(emax, fabs(scale * s));
                     }
+            if (emax >= prim_eps) {
+                prim[poff[pid] + k] = make_double4(p, P[0], P[1], P[2]);
+                k++;
+            }
         }
+    nprim_out[pid] = k;
 }
 
 // counts[k] = number of ket pairs j (sorted by Schwarz value, descending)
@@ -136,10 +146,11 @@ __global__ void fill_qlist_diag_kernel(int n, const int* __restrict__ ids, int2*
 // ---- host launchers -------------------------------------------------------
 
 int gtf_build_pairs(hipStream_t st, const BasisView& b, int npairs, const int* s1, const int* s2,
-                    const int* poff, const long* eoff, double4* prim, double* E)
+                    const int* poff, const long* eoff, double4* prim, double* E, int* nprim_out, double prim_eps)
 {
     if (npairs <= 0) return 0;
-    build_pairs_kernel<<<(npairs + 127) / 128, 128, 0, st>>>(b, npairs, s1, s2, poff, eoff, prim, E);
+    build_pairs_kernel<<<(npairs + 127) / 128, 128, 0, st>>>(b, npairs, s1, s2, poff, eoff, prim, E, nprim_out,
+                                                           prim_eps);
     GTF_CHECK(hipGetLastError());
     return 0;
 }
