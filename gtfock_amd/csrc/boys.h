@@ -39,7 +39,7 @@ __device__ __forceinline__ void boys_eval(Real T, Real* F, const double* __restr
             });
         }
     } else {
-        Real rt = Real(1.0) / sqrt(T);
+        Real rt = rsqrt_nr(T);
         F[0] = Real(0.88622692545275801365) * rt;  // sqrt(pi)/2
         if constexpr (M > 0) {
             Real e = exp(-T);

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <vector>
 
 #include "screen.h"
 
@@ -188,18 +189,40 @@ int GtfSystem::screen(double tolscr)
             if (h_v[id] > eta) s.push_back(id);
         std::sort(s.begin(), s.end(), [&](int x, int y) { return h_v[x] != h_v[y] ? h_v[x] > h_v[y] : x < y; });
         nsig[t] = (int)s.size();
-        std::vector<double> sv(s.size());
-        for (size_t k = 0; k < s.size(); k++) sv[k] = h_v[s[k]];
+        std::vector<double> sorted(s.size());
+        for (size_t k = 0; k < s.size(); k++) sorted[k] = h_v[s[k]];
+        // value rank, then each block re-sorted by primitive count (descending)
+        std::vector<int> order(s.size());
+        std::iota(order.begin(), order.end(), 0);
+        for (size_t b0 = 0; b0 < s.size(); b0 += GTF_KET_BLOCK) {
+            const size_t b1 = std::min(s.size(), b0 + GTF_KET_BLOCK);
+            std::stable_sort(order.begin() + b0, order.begin() + b1,
+                             [&](int x, int y) { return h_pnprim[s[x]] > h_pnprim[s[y]]; });
+        }
+        std::vector<int> perm(s.size());
+        h_sigv[t].resize(s.size());
+        h_rank[t].resize(s.size());
+        for (size_t k = 0; k < s.size(); k++) {
+            perm[k] = s[order[k]];
+            h_rank[t][k] = order[k];
+            h_sigv[t][k] = sorted[order[k]];
+        }
+        s = perm;
         d_sig[t] = dalloc<int>(s.size());
         d_sigv[t] = dalloc<double>(s.size());
-        if (!d_sig[t] || !d_sigv[t]) return -2;
-        if (upload(d_sig[t], s, stream) || upload(d_sigv[t], sv, stream)) return -1;
+        d_rank[t] = dalloc<int>(s.size());
+        d_sigv_sorted[t] = dalloc<double>(s.size());
+        if (!d_sig[t] || !d_sigv[t] || !d_rank[t] || !d_sigv_sorted[t]) return -2;
+        if (upload(d_sig[t], s, stream) || upload(d_sigv[t], h_sigv[t], stream) ||
+            upload(d_rank[t], h_rank[t], stream) || upload(d_sigv_sorted[t], sorted, stream))
+            return -1;
     }
     // quartet rows per class: bra type A >= ket type B
     int maxn = 1;
     for (int t = 0; t < GTF_NPAIRTYPES; t++) maxn = std::max(maxn, nsig[t]);
     long* d_counts = dalloc<long>(maxn);
-    if (!d_counts) return -2;
+    long* d_exact = dalloc<long>(maxn);
+    if (!d_counts || !d_exact) return -2;
     for (int cls = 0; cls < GTF_NCLASSES; cls++) {
         int A, B;
         gtf_class_types(cls, A, B);
@@ -207,17 +230,28 @@ int GtfSystem::screen(double tolscr)
         d_rowoff[cls] = dalloc<long>(nsig[A] + 1);
         if (!d_rowoff[cls]) return -2;
         if (nsig[A] == 0 || nsig[B] == 0) continue;
-        if (gtf_count_rows(stream, nsig[A], d_sigv[A], nsig[B], d_sigv[B], tol2, A == B, d_counts, d_rowoff[cls]))
+        if (gtf_count_rows(stream, nsig[A], d_sigv[A], d_rank[A], nsig[B], d_sigv_sorted[B], tol2, A == B,
+                           d_exact, d_counts, d_rowoff[cls]))
             return -1;
-        std::vector<long> off(nsig[A] + 1);
+        std::vector<long> off(nsig[A] + 1), exact(nsig[A]);
         SYS_CHECK(hipMemcpyAsync(off.data(), d_rowoff[cls], sizeof(long) * off.size(), hipMemcpyDeviceToHost, stream));
+        SYS_CHECK(hipMemcpyAsync(exact.data(), d_exact, sizeof(long) * exact.size(), hipMemcpyDeviceToHost, stream));
         SYS_CHECK(hipStreamSynchronize(stream));
-        nq[cls] = off[nsig[A]];
-        // primitive-quartet work of the class, for the FLOP accounting
+        nslots[cls] = off[nsig[A]];
+        // exact quartet and primitive-quartet counts (value order)
         std::vector<double> kp(nsig[B] + 1, 0.0);
-        for (int j = 0; j < nsig[B]; j++) kp[j + 1] = kp[j] + h_pnprim[h_sig[B][j]];
+        {
+            std::vector<int> byrank(nsig[B]);
+            for (int j = 0; j < nsig[B]; j++) byrank[h_rank[B][j]] = h_sig[B][j];
+            for (int j = 0; j < nsig[B]; j++) kp[j + 1] = kp[j] + h_pnprim[byrank[j]];
+        }
         double w = 0.0;
-        for (int r = 0; r < nsig[A]; r++) w += (double)h_pnprim[h_sig[A][r]] * kp[off[r + 1] - off[r]];
+        long qn = 0;
+        for (int r = 0; r < nsig[A]; r++) {
+            w += (double)h_pnprim[h_sig[A][r]] * kp[exact[r]];
+            qn += exact[r];
+        }
+        nq[cls] = qn;
         nprimq[cls] = w;
         // bra-row tiles: up to GTF_TILE_UNITS work units of one row each
         const EriClassInfo* ci = gtf_eri_class(cls);
@@ -267,14 +301,36 @@ int GtfSystem::jk(const double* D, double* J, double* K, int ld)
 {
     if (gtf_fill_int(stream, d_counters, GTF_NCLASSES, 0)) return -1;
     const PairView pv = pair_view();
-    for (int cls = 0; cls < GTF_NCLASSES; cls++) {
+    if (!cstream[0]) {
+        for (int s = 0; s < NSTREAMS; s++) {
+            SYS_CHECK(hipStreamCreate(&cstream[s]));
+            SYS_CHECK(hipEventCreateWithFlags(&join_ev[s], hipEventDisableTiming));
+        }
+        SYS_CHECK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+    }
+    SYS_CHECK(hipEventRecord(fork_ev, stream));
+    for (int s = 0; s < NSTREAMS; s++) SYS_CHECK(hipStreamWaitEvent(cstream[s], fork_ev, 0));
+    // classes in decreasing order of work, dealt round-robin over the streams
+    int order[GTF_NCLASSES];
+    for (int c = 0; c < GTF_NCLASSES; c++) order[c] = c;
+    std::sort(order, order + GTF_NCLASSES, [&](int x, int y) { return nprimq[x] > nprimq[y]; });
+    int slot = 0;
+    for (int oi = 0; oi < GTF_NCLASSES; oi++) {
+        const int cls = order[oi];
         if (nq[cls] == 0) continue;
         int A, B;
         gtf_class_types(cls, A, B);
         const EriClassInfo* ci = gtf_eri_class(cls);
+        hipStream_t cs = cstream[slot++ % NSTREAMS];
         EriArgs a{};
-        a.nunits = nq[cls] * ci->nsplit;
+        a.nunits = nslots[cls] * ci->nsplit;
         a.row_off = d_rowoff[cls];
+        a.vbra = d_sigv[A];
+        a.vket = d_sigv[B];
+        a.rank_bra = d_rank[A];
+        a.rank_ket = d_rank[B];
+        a.tol2 = tol2;
+        a.same = A == B;
         a.nrows = nsig[A];
         a.bra_list = d_sig[A];
         a.ket_list = d_sig[B];
@@ -287,16 +343,20 @@ int GtfSystem::jk(const double* D, double* J, double* K, int ld)
         a.J = J;
         a.K = K;
         a.ld = ld;
-        if (timing) SYS_CHECK(hipEventRecord(tev[2 * cls], stream));
+        if (timing) SYS_CHECK(hipEventRecord(tev[2 * cls], cs));
         if (ntiles[cls] > 0) {
             a.tiles = d_tiles[cls];
             a.ntiles = ntiles[cls];
             a.nbf = nbf;
-            if (launch(ci->jk_tile, a, std::min(a.ntiles, 2048), stream, tile_lds[cls])) return -1;
-        } else if (launch(ci->jk, a, std::min(a.ntasks, 2048), stream)) {
+            if (launch(ci->jk_tile, a, std::min(a.ntiles, 2048), cs, tile_lds[cls])) return -1;
+        } else if (launch(ci->jk, a, std::min(a.ntasks, 2048), cs)) {
             return -1;
         }
-        if (timing) SYS_CHECK(hipEventRecord(tev[2 * cls + 1], stream));
+        if (timing) SYS_CHECK(hipEventRecord(tev[2 * cls + 1], cs));
+    }
+    for (int s = 0; s < NSTREAMS; s++) {
+        SYS_CHECK(hipEventRecord(join_ev[s], cstream[s]));
+        SYS_CHECK(hipStreamWaitEvent(stream, join_ev[s], 0));
     }
     return 0;
 }
@@ -331,6 +391,11 @@ int GtfSystem::store(int cls, int nq_, const int2* d_qlist, double* d_out)
 
 void GtfSystem::release()
 {
+    for (int s = 0; s < NSTREAMS; s++) {
+        if (cstream[s]) { (void)hipStreamSynchronize(cstream[s]); (void)hipStreamDestroy(cstream[s]); cstream[s] = nullptr; }
+        if (join_ev[s]) { (void)hipEventDestroy(join_ev[s]); join_ev[s] = nullptr; }
+    }
+    if (fork_ev) { (void)hipEventDestroy(fork_ev); fork_ev = nullptr; }
     for (hipEvent_t& e : tev)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
     for (void* p : allocs) (void)hipFree(p);

@@ -36,10 +36,23 @@ struct GtfSystem {
     std::vector<int> h_sig[GTF_NPAIRTYPES];
     int* d_sig[GTF_NPAIRTYPES] = {nullptr};
     double* d_sigv[GTF_NPAIRTYPES] = {nullptr};
+    // the lists are value-sorted, then each block of GTF_KET_BLOCK entries is
+    // re-sorted by primitive count; rank = position in the value order and
+    // sigv_sorted = the values in value order (for the row counts)
+    std::vector<double> h_sigv[GTF_NPAIRTYPES];
+    std::vector<int> h_rank[GTF_NPAIRTYPES];
+    int* d_rank[GTF_NPAIRTYPES] = {nullptr};
+    double* d_sigv_sorted[GTF_NPAIRTYPES] = {nullptr};
     // quartet rows per class
-    long nq[GTF_NCLASSES] = {0};
+    long nq[GTF_NCLASSES] = {0};      // screened unique quartets
+    long nslots[GTF_NCLASSES] = {0};  // row slots incl. the block padding
     long* d_rowoff[GTF_NCLASSES] = {nullptr};
     int* d_counters = nullptr;
+    // class kernels run concurrently on these streams (forked from and joined
+    // to the caller's stream with events)
+    static constexpr int NSTREAMS = 4;
+    hipStream_t cstream[NSTREAMS] = {nullptr};
+    hipEvent_t fork_ev = nullptr, join_ev[NSTREAMS] = {nullptr};
     size_t dev_bytes = 0;
     // bra-row tiles per class for the LDS-strip kernel (empty: global-atomic kernel)
     int ntiles[GTF_NCLASSES] = {0};

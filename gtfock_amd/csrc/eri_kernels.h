@@ -45,7 +45,21 @@ struct EriArgs {
     const int4* tiles;
     int ntiles;
     int nbf;
+    // exact screening of the block-padded rows
+    const double* vbra;
+    const double* vket;
+    const int* rank_bra;
+    const int* rank_ket;
+    double tol2;
+    int same;
 };
+
+// the Schwarz test of pfock/fock_task.c:192 and, for a diagonal class, the
+// one-orientation rule, for bra row r and ket slot j
+__device__ __forceinline__ bool quartet_valid(const EriArgs& a, int r, int j)
+{
+    return a.vbra[r] * a.vket[j] >= a.tol2 && (!a.same || a.rank_ket[j] <= a.rank_bra[r]);
+}
 
 __device__ __forceinline__ int find_row(const long* __restrict__ off, int nrows, long q)
 {
@@ -83,8 +97,10 @@ __device__ __forceinline__ void eri_unit(const EriArgs& a, long u, const double*
         ket = bk.y;
     } else {
         const int r = find_row(a.row_off, a.nrows, q);
+        const int j = (int)(q - a.row_off[r]);
+        if (!quartet_valid(a, r, j)) return;
         bra = a.bra_list[r];
-        ket = a.ket_list[q - a.row_off[r]];
+        ket = a.ket_list[j];
     }
     double acc[NC][NCD];
     eri_block<LA, LB, LC, LD, NC, double, BSTRIDE>(a.pv, bra, ket, c0, boys, acc);
@@ -280,6 +296,7 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
         for (int u = threadIdx.x; u < nunits; u += 256) {
             const int q = u / NSPLIT;
             const int c0 = (u - q * NSPLIT) * NC;
+            if (!quartet_valid(a, t.x, t.y + q)) continue;
             const int ket = a.ket_list[t.y + q];
             double acc[NC][NCD];
             eri_block<LA, LB, LC, LD, NC, double, BOYS_NCOL>(a.pv, bra, ket, c0, a.boys, acc);

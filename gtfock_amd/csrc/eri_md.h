@@ -116,13 +116,12 @@ __device__ __forceinline__ void eri_block(const PairView& pv, int bra, int ket, 
             const double4 K = pk[j];
             const Real p = Real(B.x), q = Real(K.x);
             const Real X = Real(B.y - K.y), Y = Real(B.z - K.z), Z = Real(B.w - K.w);
-            const Real s = p + q;
-            const Real rs = Real(1) / s;
-            const Real alpha = p * q * rs;
+            const Real r = rsqrt_nr(p + q);
+            const Real alpha = p * q * (r * r);
             const Real T = alpha * (X * X + Y * Y + Z * Z);
             Real F[L + 1];
             boys_eval<L, Real, BSTRIDE>(T, F, boys_tab);
-            const Real pref = Real(GTF_TWO_PI_2_5) * sqrt(rs);
+            const Real pref = Real(GTF_TWO_PI_2_5) * r;
             Real R[HL];
             hermite_R<L, Real>(R, F, pref, Real(-2) * alpha, X, Y, Z);
             Real V[NC][HCD];

@@ -17,6 +17,8 @@
 #define GTF_LMAX 2
 #define GTF_NPAIRTYPES 6
 #define GTF_NCLASSES 21
+// ket-list block re-sorted by primitive count (one wavefront)
+#define GTF_KET_BLOCK 64
 
 // Boys-function grid: F_m(T0) for T0 = i * BOYS_DELTA, i = 0..BOYS_NGRID-1,
 // m = 0..BOYS_NCOL-1; T >= BOYS_TMAX uses the asymptotic form.
@@ -74,6 +76,24 @@ __host__ __device__ constexpr int herm_v(int h)
     return r - s * (s + 1) / 2;
 }
 __host__ __device__ constexpr int herm_u(int h) { return horder(h) - herm_t(h) - herm_v(h); }
+
+// 1/sqrt(s) from the hardware estimate refined by two Newton steps
+// y <- y (3 - s y^2) / 2 (relative error ~1e-16 in FP64)
+__device__ __forceinline__ double rsqrt_nr(double s)
+{
+    double y = __builtin_amdgcn_rsq(s);
+    double h = s * y;
+    y = fma(y, fma(-0.5 * h, y, 0.5), y);
+    h = s * y;
+    y = fma(y, fma(-0.5 * h, y, 0.5), y);
+    return y;
+}
+__device__ __forceinline__ float rsqrt_nr(float s)
+{
+    float y = __builtin_amdgcn_rsqf(s);
+    y = fmaf(y, fmaf(-0.5f * s * y, y, 0.5f), y);
+    return y;
+}
 
 // compile-time loop: f(std::integral_constant<int, i>) for i in [B, E)
 template <int B, int E, typename F>

@@ -546,7 +546,9 @@ long gtf_pfock_quartets(PFock_t pf, int* out, long cap)
         if (hipMemcpy(off.data(), s.d_rowoff[cls], sizeof(long) * off.size(), hipMemcpyDeviceToHost) != hipSuccess)
             return -1;
         for (int r = 0; r < s.nsig[A]; r++)
-            for (long j = 0; j < off[r + 1] - off[r]; j++, k++) {
+            for (long j = 0; j < off[r + 1] - off[r]; j++) {
+                if (!(s.h_sigv[A][r] * s.h_sigv[B][j] >= s.tol2) || (A == B && s.h_rank[B][j] > s.h_rank[A][r]))
+                    continue;
                 if (out && k < cap) {
                     const int bra = s.h_sig[A][r], ket = s.h_sig[B][j];
                     out[4 * k + 0] = s.h_s1[bra];
@@ -554,6 +556,7 @@ long gtf_pfock_quartets(PFock_t pf, int* out, long cap)
                     out[4 * k + 2] = s.h_s1[ket];
                     out[4 * k + 3] = s.h_s2[ket];
                 }
+                k++;
             }
     }
     return k;

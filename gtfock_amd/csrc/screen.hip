@@ -89,12 +89,14 @@ __global__ void build_pairs_kernel(BasisView b, int npairs, const int* __restric
     nprim_out[pid] = k;
 }
 
-// counts[k] = number of ket pairs j (sorted by Schwarz value, descending)
-// with v_bra[k] * v_ket[j] >= tol2 (the test of pfock/fock_task.c:192); for a
-// diagonal class only j <= k, so each unordered pair of pairs appears once.
-__global__ void count_rows_kernel(int nbra, const double* __restrict__ vbra, int nket,
-                                  const double* __restrict__ vket, double tol2, int same,
-                                  long* __restrict__ counts)
+// exact[k] = number of ket pairs j (in value order) with
+// v_bra[k] * v_ket[j] >= tol2 (the test of pfock/fock_task.c:192); for a
+// diagonal class only rank(j) <= rank(k), so each unordered pair of pairs
+// appears once.  counts[k] rounds exact[k] up to whole ket blocks (the kernel
+// re-applies the exact test to the padding).
+__global__ void count_rows_kernel(int nbra, const double* __restrict__ vbra, const int* __restrict__ rank_bra,
+                                  int nket, const double* __restrict__ vket_sorted, double tol2, int same,
+                                  long* __restrict__ exact, long* __restrict__ counts)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nbra) return;
@@ -102,12 +104,14 @@ __global__ void count_rows_kernel(int nbra, const double* __restrict__ vbra, int
     int lo = 0, hi = nket;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (vk * vket[mid] >= tol2) lo = mid + 1;
+        if (vk * vket_sorted[mid] >= tol2) lo = mid + 1;
         else hi = mid;
     }
     long cnt = lo;
-    if (same && cnt > k + 1) cnt = k + 1;
-    counts[k] = cnt;
+    if (same && cnt > rank_bra[k] + 1) cnt = rank_bra[k] + 1;
+    exact[k] = cnt;
+    long pad = cnt == 0 ? 0 : ((cnt + GTF_KET_BLOCK - 1) / GTF_KET_BLOCK) * GTF_KET_BLOCK;
+    counts[k] = pad < nket ? pad : nket;
 }
 
 // Exclusive prefix sum of n longs in one workgroup of 1024 threads:
@@ -155,11 +159,12 @@ int gtf_build_pairs(hipStream_t st, const BasisView& b, int npairs, const int* s
     return 0;
 }
 
-int gtf_count_rows(hipStream_t st, int nbra, const double* vbra, int nket, const double* vket, double tol2,
-                   int same, long* counts, long* row_off)
+int gtf_count_rows(hipStream_t st, int nbra, const double* vbra, const int* rank_bra, int nket,
+                   const double* vket_sorted, double tol2, int same, long* exact, long* counts, long* row_off)
 {
     if (nbra <= 0) return 0;
-    count_rows_kernel<<<(nbra + 255) / 256, 256, 0, st>>>(nbra, vbra, nket, vket, tol2, same, counts);
+    count_rows_kernel<<<(nbra + 255) / 256, 256, 0, st>>>(nbra, vbra, rank_bra, nket, vket_sorted, tol2, same,
+                                                          exact, counts);
     GTF_CHECK(hipGetLastError());
     scan_kernel<<<1, 1024, 0, st>>>(counts, row_off, nbra);
     GTF_CHECK(hipGetLastError());
