@@ -95,6 +95,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c10h22")
     ap.add_argument("--tol", type=float, default=1e-11)
+    ap.add_argument("--tau", type=float, default=1e-4, help="Schwarz-bound threshold of the FP32 ERIs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--classes", action="store_true", help="print the per-class breakdown to stderr")
     args = ap.parse_args()
@@ -157,6 +158,29 @@ def main():
                 "algorithmic_flops_per_launch": dom_fl, "avg_launch_ms": dom_s * 1e3,
                 "build_tflops": total_fl / (ms_step * 1e-3) / 1e12}
 
+    # mixed precision on the same molecule and density: FP32 ERIs for quartets
+    # whose Schwarz bound is below tau, FP64 J/K accumulation
+    F64 = pf.get_mat(g.MAT_F)
+    pf.set_timing(False)
+    pf.set_precision(args.tau)
+    for _ in range(args.warmup):
+        pf.compute_fock()
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pf.compute_fock()
+    device_sync()
+    t1 = time.perf_counter()
+    F32 = pf.get_mat(g.MAT_F)
+    nq_lo, _ = pf.mixed_counts()
+    mixed = {"value": args.steps / (t1 - t0), "unit": "Fock builds/s", "tau": args.tau,
+             "ms_per_step": (t1 - t0) * 1e3 / args.steps,
+             "speedup_vs_fp64": (args.steps / (t1 - t0)) / value,
+             "fp32_quartet_fraction": float(nq_lo.sum()) / max(1.0, float(nq.sum())),
+             "max_abs_dF": float(np.abs(F32 - F64).max()),
+             "dE_Eh": float(abs(np.sum(D * (F32 - F64))))}
+    pf.set_precision(0.0)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(gbs, xyz, D, args.tol)
@@ -171,6 +195,7 @@ def main():
                    "nshells": basis.nshells, "tolscr": args.tol, "quartets": int(st["usq"]),
                    "integrals": int(st["uitl"]), "create_s": round(t_create, 3)},
         "roofline": roofline,
+        "mixed": mixed,
         "cpu_baseline": cpu,
     }
     pf.close()

@@ -110,6 +110,8 @@ def lib():
         "gtf_pfock_stream": ([_vp], _vp),
         "gtf_pfock_class_work": ([_vp, _lp, _dp], ctypes.c_int),
         "gtf_pfock_set_timing": ([_vp, ctypes.c_int], ctypes.c_int),
+        "gtf_pfock_set_precision": ([_vp, ctypes.c_double], ctypes.c_int),
+        "gtf_pfock_mixed_counts": ([_vp, _lp, _dp], ctypes.c_int),
         "gtf_pfock_class_times": ([_vp, _dp, _lp], ctypes.c_int),
         "gtf_pfock_device_bytes": ([_vp], ctypes.c_double),
         "gtf_erd_quartets": ([_vp, ctypes.c_int, _ip, _dp], ctypes.c_int),
@@ -276,6 +278,16 @@ class PFock:
         lib().gtf_pfock_class_work(self.h, nq.ctypes.data_as(_lp), _p(npq))
         return nq, npq
 
+    def set_precision(self, tau):
+        """Mixed precision: FP32 for quartets with Schwarz bound < tau (0: FP64)."""
+        _check(lib().gtf_pfock_set_precision(self.h, float(tau)), "gtf_pfock_set_precision")
+
+    def mixed_counts(self):
+        nq = np.zeros(21, dtype=np.int64)
+        npq = np.zeros(21)
+        lib().gtf_pfock_mixed_counts(self.h, nq.ctypes.data_as(_lp), _p(npq))
+        return nq, npq
+
     def set_timing(self, on=True):
         _check(lib().gtf_pfock_set_timing(self.h, 1 if on else 0), "gtf_pfock_set_timing")
 
@@ -387,11 +399,14 @@ class PurifT(ctypes.Structure):
         ("timepass", ctypes.c_double), ("dev", _vp)]
 
 
-def scf(gbs, xyz, niters, tolscr=1e-11, verbose=False):
+def scf(gbs, xyz, niters, tolscr=1e-11, verbose=False, tau=0.0):
     """pscf/scf.c:189-474 on one GPU with a zero initial density
-    (core-Hamiltonian guess).  Returns (energies, last D, last F = H + G(D))."""
+    (core-Hamiltonian guess).  tau > 0 selects the mixed-precision Fock
+    build.  Returns (energies, last D, last F = H + G(D))."""
     basis = BasisSet(gbs, xyz)
     pf = PFock(basis, tolscr=tolscr)
+    if tau > 0:
+        pf.set_precision(tau)
     pu = Purif(basis)
     S = pf.overlap()
     X = pf.x_matrix()

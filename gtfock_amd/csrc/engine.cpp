@@ -101,7 +101,7 @@ int GtfSystem::init(const BasisSet* b, hipStream_t st)
     d_peoff = dalloc<long>(npairs);
     d_prim = dalloc<double4>(np);
     d_E = dalloc<double>(ne);
-    d_counters = dalloc<int>(GTF_NCLASSES);
+    d_counters = dalloc<int>(2 * GTF_NCLASSES);
     if (!d_s1 || !d_s2 || !d_pnprim || !d_ppoff || !d_peoff || !d_prim || !d_E || !d_counters) return -2;
     if (upload(d_s1, h_s1, st) || upload(d_s2, h_s2, st) || upload(d_pnprim, h_pnprim, st) ||
         upload(d_ppoff, ppoff, st) || upload(d_peoff, peoff, st))
@@ -253,24 +253,11 @@ int GtfSystem::screen(double tolscr)
         }
         nq[cls] = qn;
         nprimq[cls] = w;
-        // bra-row tiles: up to GTF_TILE_UNITS work units of one row each
-        const EriClassInfo* ci = gtf_eri_class(cls);
-        const int NA = 2 * ci->la + 1, NB = 2 * ci->lb + 1;
-        tile_lds[cls] = sizeof(double) * ((size_t)(NA + NB) * nbf + NA * NB);
-        if (use_tiles && tile_lds[cls] <= max_lds) {
-            const int tq = std::max(1, GTF_TILE_UNITS / ci->nsplit);
-            std::vector<int4> tiles;
-            for (int r = 0; r < nsig[A]; r++) {
-                const long cnt = off[r + 1] - off[r];
-                for (long j0 = 0; j0 < cnt; j0 += tq)
-                    tiles.push_back(make_int4(r, (int)j0, (int)std::min<long>(cnt, j0 + tq), 0));
-            }
-            ntiles[cls] = (int)tiles.size();
-            d_tiles[cls] = dalloc<int4>(tiles.size());
-            if (!d_tiles[cls]) return -2;
-            if (upload(d_tiles[cls], tiles, stream)) return -1;
-        }
+        h_exact[cls] = exact;
+        h_slots[cls].resize(nsig[A]);
+        for (int r = 0; r < nsig[A]; r++) h_slots[cls][r] = off[r + 1] - off[r];
     }
+    if (build_tiles()) return -1;
     screened = true;
     return 0;
 }
@@ -297,9 +284,73 @@ void GtfSystem::csr(std::vector<int>& shellptr, std::vector<int>& shellid, std::
     shellptr[nshells] = (int)shellid.size();
 }
 
+// Tiles of up to GTF_TILE_UNITS work units of one bra row.  With tau > 0 each
+// row splits at the first ket whose Schwarz bound sqrt(v_bra v_ket) < tau:
+// FP64 tiles cover the block-padded prefix, FP32 tiles start at the block
+// holding the split; the kernels re-test the exact predicates per quartet.
+int GtfSystem::build_tiles()
+{
+    const double tau2 = tau * tau;
+    for (int cls = 0; cls < GTF_NCLASSES; cls++) {
+        int A, B;
+        gtf_class_types(cls, A, B);
+        const EriClassInfo* ci = gtf_eri_class(cls);
+        const int NA = 2 * ci->la + 1, NB = 2 * ci->lb + 1;
+        tile_lds[cls] = sizeof(double) * ((size_t)(NA + NB) * nbf + NA * NB);
+        ntiles[cls] = ntiles_lo[cls] = 0;
+        nq_lo[cls] = 0;
+        nprimq_lo[cls] = 0.0;
+        if (nq[cls] == 0 || !use_tiles || tile_lds[cls] > max_lds) continue;
+        std::vector<double> vs(nsig[B]);
+        std::vector<int> byrank(nsig[B]);
+        for (int j = 0; j < nsig[B]; j++) {
+            vs[h_rank[B][j]] = h_sigv[B][j];
+            byrank[h_rank[B][j]] = h_sig[B][j];
+        }
+        std::vector<double> kp(nsig[B] + 1, 0.0);
+        for (int j = 0; j < nsig[B]; j++) kp[j + 1] = kp[j] + h_pnprim[byrank[j]];
+        const int tq = std::max(1, GTF_TILE_UNITS / ci->nsplit);
+        std::vector<int4> hi, lo;
+        for (int r = 0; r < nsig[A]; r++) {
+            const long c = h_exact[cls][r];
+            if (c == 0) continue;
+            long c64 = c;
+            if (tau2 > 0.0) {
+                const double vk = h_sigv[A][r];
+                c64 = std::upper_bound(vs.begin(), vs.begin() + c, vk, [&](double v, double x) {
+                          return !(v * x >= tau2);
+                      }) - vs.begin();
+            }
+            const long pad = std::min<long>(nsig[B], ((c + GTF_KET_BLOCK - 1) / GTF_KET_BLOCK) * GTF_KET_BLOCK);
+            const long pad64 = std::min<long>(nsig[B], ((c64 + GTF_KET_BLOCK - 1) / GTF_KET_BLOCK) * GTF_KET_BLOCK);
+            for (long j0 = 0; j0 < pad64; j0 += tq)
+                hi.push_back(make_int4(r, (int)j0, (int)std::min<long>(pad64, j0 + tq), 0));
+            if (c > c64) {
+                const long s0 = (c64 / GTF_KET_BLOCK) * GTF_KET_BLOCK;
+                for (long j0 = s0; j0 < pad; j0 += tq)
+                    lo.push_back(make_int4(r, (int)j0, (int)std::min<long>(pad, j0 + tq), 0));
+                nq_lo[cls] += c - c64;
+                nprimq_lo[cls] += (double)h_pnprim[h_sig[A][r]] * (kp[c] - kp[c64]);
+            }
+        }
+        ntiles[cls] = (int)hi.size();
+        ntiles_lo[cls] = (int)lo.size();
+        if (!hi.empty()) {
+            d_tiles[cls] = dalloc<int4>(hi.size());
+            if (!d_tiles[cls] || upload(d_tiles[cls], hi, stream)) return -1;
+        }
+        if (!lo.empty()) {
+            d_tiles_lo[cls] = dalloc<int4>(lo.size());
+            if (!d_tiles_lo[cls] || upload(d_tiles_lo[cls], lo, stream)) return -1;
+        }
+    }
+    SYS_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
 int GtfSystem::jk(const double* D, double* J, double* K, int ld)
 {
-    if (gtf_fill_int(stream, d_counters, GTF_NCLASSES, 0)) return -1;
+    if (gtf_fill_int(stream, d_counters, 2 * GTF_NCLASSES, 0)) return -1;
     const PairView pv = pair_view();
     if (!cstream[0]) {
         for (int s = 0; s < NSTREAMS; s++) {
@@ -344,11 +395,21 @@ int GtfSystem::jk(const double* D, double* J, double* K, int ld)
         a.K = K;
         a.ld = ld;
         if (timing) SYS_CHECK(hipEventRecord(tev[2 * cls], cs));
-        if (ntiles[cls] > 0) {
-            a.tiles = d_tiles[cls];
-            a.ntiles = ntiles[cls];
+        if (ntiles[cls] > 0 || ntiles_lo[cls] > 0) {
             a.nbf = nbf;
-            if (launch(ci->jk_tile, a, std::min(a.ntiles, 2048), cs, tile_lds[cls])) return -1;
+            a.tau2 = tau * tau;
+            if (ntiles[cls] > 0) {
+                a.tiles = d_tiles[cls];
+                a.ntiles = ntiles[cls];
+                a.counter = d_counters + cls;
+                if (launch(ci->jk_tile, a, std::min(a.ntiles, 2048), cs, tile_lds[cls])) return -1;
+            }
+            if (ntiles_lo[cls] > 0) {
+                a.tiles = d_tiles_lo[cls];
+                a.ntiles = ntiles_lo[cls];
+                a.counter = d_counters + GTF_NCLASSES + cls;
+                if (launch(ci->jk_tile_lo, a, std::min(a.ntiles, 2048), cs, tile_lds[cls])) return -1;
+            }
         } else if (launch(ci->jk, a, std::min(a.ntasks, 2048), cs)) {
             return -1;
         }

@@ -54,9 +54,18 @@ struct GtfSystem {
     hipStream_t cstream[NSTREAMS] = {nullptr};
     hipEvent_t fork_ev = nullptr, join_ev[NSTREAMS] = {nullptr};
     size_t dev_bytes = 0;
-    // bra-row tiles per class for the LDS-strip kernel (empty: global-atomic kernel)
+    // bra-row tiles per class for the LDS-strip kernels (empty: global-atomic
+    // kernel); tiles[cls] are the FP64 tiles, tiles_lo[cls] the FP32 ones
     int ntiles[GTF_NCLASSES] = {0};
     int4* d_tiles[GTF_NCLASSES] = {nullptr};
+    int ntiles_lo[GTF_NCLASSES] = {0};
+    int4* d_tiles_lo[GTF_NCLASSES] = {nullptr};
+    std::vector<long> h_exact[GTF_NCLASSES], h_slots[GTF_NCLASSES];
+    // mixed precision: Schwarz bound threshold tau (0 = plain FP64)
+    double tau = 0.0;
+    long nq_lo[GTF_NCLASSES] = {0};
+    double nprimq_lo[GTF_NCLASSES] = {0};
+    int build_tiles();
     size_t tile_lds[GTF_NCLASSES] = {0};
     size_t max_lds = 65536;
     bool use_tiles = true;

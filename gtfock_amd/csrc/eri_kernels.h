@@ -52,6 +52,8 @@ struct EriArgs {
     const int* rank_ket;
     double tol2;
     int same;
+    // mixed precision: FP64 quartets have v_bra * v_ket >= tau2, FP32 the rest
+    double tau2;
 };
 
 // the Schwarz test of pfock/fock_task.c:192 and, for a diagonal class, the
@@ -271,12 +273,13 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
 // (NA + NB) * nbf + NA * NB doubles.
 #define GTF_TILE_UNITS 1024
 
-template <int LA, int LB, int LC, int LD, int NC>
+template <int LA, int LB, int LC, int LD, int NC, typename Real>
 __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
 {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr int NSPLIT = S::NAB / NC;
     constexpr int NA = S::NA, NB = S::NB, NCD = S::NCD;
+    constexpr bool LOW = std::is_same<Real, float>::value;
     extern __shared__ double smem[];
     __shared__ int s_task;
     const int nbf = a.nbf;
@@ -296,10 +299,23 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
         for (int u = threadIdx.x; u < nunits; u += 256) {
             const int q = u / NSPLIT;
             const int c0 = (u - q * NSPLIT) * NC;
-            if (!quartet_valid(a, t.x, t.y + q)) continue;
-            const int ket = a.ket_list[t.y + q];
+            const int j = t.y + q;
+            if (!quartet_valid(a, t.x, j)) continue;
+            // precision split on the Schwarz bound sqrt(v_bra v_ket) vs tau
+            if ((a.vbra[t.x] * a.vket[j] >= a.tau2) == LOW) continue;
+            const int ket = a.ket_list[j];
             double acc[NC][NCD];
-            eri_block<LA, LB, LC, LD, NC, double, BOYS_NCOL>(a.pv, bra, ket, c0, a.boys, acc);
+            if constexpr (LOW) {
+                float accf[NC][NCD];
+                eri_block<LA, LB, LC, LD, NC, float, BOYS_NCOL>(a.pv, bra, ket, c0, a.boys, accf);
+                static_for<0, NC>([&](auto c_) {
+                    static_for<0, NCD>([&](auto d_) {
+                        acc[decltype(c_)::value][decltype(d_)::value] = (double)accf[decltype(c_)::value][decltype(d_)::value];
+                    });
+                });
+            } else {
+                eri_block<LA, LB, LC, LD, NC, double, BOYS_NCOL>(a.pv, bra, ket, c0, a.boys, acc);
+            }
             digest<LA, LB, LC, LD, NC, true>(a, bra, ket, c0, acc, jmn, strip, nbf, 0);
         }
         __syncthreads();
@@ -329,6 +345,7 @@ struct EriClassInfo {
     int nsplit;     // threads per quartet
     int nints;      // integrals per quartet
     eri_kernel_t jk, diag, store;
-    eri_kernel_t jk_tile;
+    eri_kernel_t jk_tile;     // FP64 quartets (v_bra v_ket >= tau2)
+    eri_kernel_t jk_tile_lo;  // FP32 quartets (v_bra v_ket < tau2)
 };
 const EriClassInfo* gtf_eri_class(int cls);

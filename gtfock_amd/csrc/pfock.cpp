@@ -583,6 +583,23 @@ int gtf_pfock_class_work(PFock_t pf, long* nquartets, double* nprimquartets)
     return GTF_NCLASSES;
 }
 
+int gtf_pfock_set_precision(PFock_t pf, double tau)
+{
+    if (tau < 0.0) return -1;
+    if (hipStreamSynchronize(pf->stream) != hipSuccess) return -1;
+    pf->sys.tau = tau;
+    return pf->sys.build_tiles();
+}
+
+int gtf_pfock_mixed_counts(PFock_t pf, long* nq_lo, double* nprimq_lo)
+{
+    for (int c = 0; c < GTF_NCLASSES; c++) {
+        nq_lo[c] = pf->sys.nq_lo[c];
+        nprimq_lo[c] = pf->sys.nprimq_lo[c];
+    }
+    return GTF_NCLASSES;
+}
+
 int gtf_pfock_set_timing(PFock_t pf, int on)
 {
     GtfSystem& s = pf->sys;

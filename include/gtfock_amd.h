@@ -41,6 +41,13 @@ int gtf_pfock_device_ptrs(PFock_t pfock, int index, double **D, double **J, doub
  * nprim(bra) * nprim(ket)), the work units of the FLOP accounting. */
 int gtf_pfock_class_work(PFock_t pfock, long *nquartets, double *nprimquartets);
 
+/* Mixed precision: quartets whose Schwarz bound sqrt(v_MN v_PQ) is below
+ * tau are evaluated in FP32 (FP64 J/K accumulation); tau = 0 restores the
+ * plain FP64 build.  Takes effect from the next PFock_computeFock. */
+int gtf_pfock_set_precision(PFock_t pfock, double tau);
+/* Per class: quartets and primitive quartets evaluated in FP32. */
+int gtf_pfock_mixed_counts(PFock_t pfock, long *nquartets_fp32, double *nprimquartets_fp32);
+
 /* Per-class kernel timing with HIP events recorded on the engine's stream
  * around each class launch; enabling resets the accumulators. */
 int gtf_pfock_set_timing(PFock_t pfock, int on);
