@@ -120,6 +120,22 @@ def lib():
         "destroy_purif": ([_vp], None),
         "compute_purification": ([_vp, _dp, _dp], ctypes.c_int),
         "compute_diis": ([_vp, _vp, _dp, _dp, ctypes.c_int], None),
+        "gtf_batch_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_double, ctypes.POINTER(_vp)],
+                             ctypes.c_int),
+        "gtf_batch_destroy": ([_vp], None),
+        "gtf_batch_layout": ([_vp, _ip, _lp], ctypes.c_int),
+        "gtf_batch_total": ([_vp], ctypes.c_long),
+        "gtf_batch_set_precision": ([_vp, ctypes.c_double], ctypes.c_int),
+        "gtf_batch_use_graph": ([_vp, ctypes.c_int], ctypes.c_int),
+        "gtf_batch_put_density": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+        "gtf_batch_compute_fock": ([_vp], ctypes.c_int),
+        "gtf_batch_get_fock": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+        "gtf_batch_get_density": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+        "gtf_batch_device_ptrs": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], ctypes.c_int),
+        "gtf_batch_stream": ([_vp], _vp),
+        "gtf_batch_stats": ([_vp, _dp], ctypes.c_int),
+        "gtf_batch_class_work": ([_vp, _lp, _dp, _lp, _dp], ctypes.c_int),
+        "gtf_batch_scf": ([_vp, ctypes.c_int, ctypes.c_double, _dp, _ip, _dp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -385,6 +401,110 @@ class Purif:
         if it < 0:
             raise GtfError("compute_purification failed")
         return D, it
+
+
+class Batch:
+    """Many molecules per Fock build and per SCF iteration (gtf_batch_*).
+
+    Matrices are packed: molecule m is the nbf[m] x nbf[m] block at
+    offset[m] of one array.  ``fock(Ds)`` takes and returns a list of
+    per-molecule matrices; G = 2J(D) - K(D) as PFock_getMat(F)."""
+
+    def __init__(self, bases, tolscr=1e-11, tau=0.0, graph=True):
+        L = lib()
+        self.bases = list(bases)
+        self.nmol = len(self.bases)
+        arr = (_vp * self.nmol)(*[b.h for b in self.bases])
+        self.h = _vp()
+        st = L.gtf_batch_create(arr, self.nmol, tolscr, ctypes.byref(self.h))
+        if st != 0:
+            self.h = None
+            raise GtfError("gtf_batch_create failed (%d)" % st)
+        self.nbf = np.zeros(self.nmol, dtype=np.int32)
+        self.offset = np.zeros(self.nmol, dtype=np.int64)
+        L.gtf_batch_layout(self.h, self.nbf.ctypes.data_as(_ip), self.offset.ctypes.data_as(_lp))
+        self.total = int(L.gtf_batch_total(self.h))
+        if tau > 0:
+            self.set_precision(tau)
+        if not graph:
+            L.gtf_batch_use_graph(self.h, 0)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gtf_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _st(self, st, what):
+        if st != 0:
+            raise GtfError("%s failed (%d)" % (what, st))
+
+    def set_precision(self, tau):
+        self._st(lib().gtf_batch_set_precision(self.h, tau), "gtf_batch_set_precision")
+
+    def use_graph(self, on):
+        self._st(lib().gtf_batch_use_graph(self.h, 1 if on else 0), "gtf_batch_use_graph")
+
+    def pack(self, mats):
+        out = np.zeros(self.total)
+        for m, A in enumerate(mats):
+            n = int(self.nbf[m])
+            out[self.offset[m]:self.offset[m] + n * n] = np.ascontiguousarray(A, dtype=np.float64).reshape(-1)
+        return out
+
+    def unpack(self, packed):
+        return [packed[self.offset[m]:self.offset[m] + int(self.nbf[m]) ** 2].reshape(int(self.nbf[m]),
+                                                                                       int(self.nbf[m])).copy()
+                for m in range(self.nmol)]
+
+    def put_density(self, Ds):
+        buf = self.pack(Ds)
+        self._st(lib().gtf_batch_put_density(self.h, -1, _p(buf)), "gtf_batch_put_density")
+
+    def compute(self):
+        self._st(lib().gtf_batch_compute_fock(self.h), "gtf_batch_compute_fock")
+
+    def get_fock(self):
+        buf = np.zeros(self.total)
+        self._st(lib().gtf_batch_get_fock(self.h, -1, _p(buf)), "gtf_batch_get_fock")
+        return self.unpack(buf)
+
+    def get_density(self):
+        buf = np.zeros(self.total)
+        self._st(lib().gtf_batch_get_density(self.h, -1, _p(buf)), "gtf_batch_get_density")
+        return self.unpack(buf)
+
+    def fock(self, Ds):
+        self.put_density(Ds)
+        self.compute()
+        return self.get_fock()
+
+    def stats(self):
+        out = np.zeros(11)
+        lib().gtf_batch_stats(self.h, _p(out))
+        keys = ["nquartets", "nprimquartets", "nq_fp32", "npq_fp32", "gpu_ms", "graph", "device_bytes",
+                "create_s", "scf_fock_s", "scf_diis_s", "scf_purif_s"]
+        return dict(zip(keys, out.tolist()))
+
+    def class_work(self):
+        nq = np.zeros(21, dtype=np.int64)
+        npq = np.zeros(21)
+        nq32 = np.zeros(21, dtype=np.int64)
+        npq32 = np.zeros(21)
+        lib().gtf_batch_class_work(self.h, nq.ctypes.data_as(_lp), _p(npq), nq32.ctypes.data_as(_lp), _p(npq32))
+        return nq, npq, nq32, npq32
+
+    def scf(self, max_iters=30, ediff=1e-11, history=False):
+        """Batched RHF SCF; returns (energies, builds used[, history])."""
+        e = np.zeros(self.nmol)
+        it = np.zeros(self.nmol, dtype=np.int32)
+        hist = np.full(self.nmol * max_iters, np.nan)
+        self._st(lib().gtf_batch_scf(self.h, max_iters, ediff, _p(e), it.ctypes.data_as(_ip), _p(hist)),
+                 "gtf_batch_scf")
+        if history:
+            return e, it, hist.reshape(self.nmol, max_iters)
+        return e, it
 
 
 class PurifT(ctypes.Structure):

@@ -1,0 +1,629 @@
+// batch.cpp -- many molecules per Fock build and per SCF iteration.
+//
+// A batch is one multi-molecule GtfSystem: the screened quartets of every
+// molecule go into the same per-class tile lists, so one launch per quartet
+// class (one captured HIP graph per build) serves the whole batch.  The SCF
+// driver around it is pscf/scf.c:189-474 per molecule, with the dense steps
+// (DIIS, pscf/purif.c:491-641, and purification, pscf/purif.c:257-488) run
+// for all molecules by the kernels of batch_linalg.hip; each molecule keeps
+// its own iteration count and convergence test.
+#include <sys/time.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/gtfock_amd.h"
+#include "batch_linalg.h"
+#include "engine.h"
+#include "oneel.h"
+#include "screen.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#define B_MAX_DIIS 10
+
+#define B_CHECK(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            GTF_PRINTF("HIP error %s (%s)\n", hipGetErrorString(e_), #call);                 \
+            return -1;                                                                       \
+        }                                                                                    \
+    } while (0)
+
+static double bwtime()
+{
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+struct GtfBatch {
+    std::vector<const BasisSet*> bases;
+    GtfSystem sys;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int nmol = 0, maxn = 0, maxpld = 0;
+    std::vector<int> n, pld, nocc, atom_off;
+    std::vector<long> off, poff;
+    long total = 0, ptotal = 0;
+    std::vector<double> enuc;
+    long *d_off = nullptr, *d_poff = nullptr;
+    int *d_n = nullptr, *d_pld = nullptr, *d_nocc = nullptr;
+    // Fock build (packed): D in, J / K accumulators, G = J + K = 2J(D) - K(D)
+    double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr, *d_G = nullptr;
+    // HIP graph of one build
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double last_ms = 0.0, create_s = 0.0;
+    long nbuilds = 0;
+    // SCF state
+    bool scf_ready = false;
+    double *d_H = nullptr, *d_S = nullptr, *d_F = nullptr;                      // packed
+    double *p_X = nullptr, *p_S = nullptr, *p_F = nullptr, *p_D = nullptr;      // padded
+    double *p_D2 = nullptr, *p_D3 = nullptr, *p_W = nullptr, *p_W2 = nullptr;  // padded work
+    double *evecs = nullptr, *Fvecs = nullptr;                                  // B_MAX_DIIS padded sets
+    double *d_stats = nullptr, *d_dots = nullptr, *d_coef = nullptr, *d_energy = nullptr;
+    int *d_iters = nullptr, *d_skip = nullptr, *d_nvec = nullptr, *d_slot = nullptr;
+    // per-molecule DIIS bookkeeping (pscf/purif.h b_mat, len_diis, bmax, bmax_id)
+    std::vector<int> len_diis, bmax_id;
+    std::vector<double> bmax, b_mat;
+    std::vector<int> purif_iters;
+    double time_fock = 0, time_diis = 0, time_purif = 0;
+
+    BatchLayout layout() const { return BatchLayout{nmol, maxn, maxpld, d_off, d_n, d_poff, d_pld}; }
+    int enqueue_fock();
+    int fock();
+    int scf_setup();
+    int diis(int iter, const std::vector<int>& active);
+    int purify(const std::vector<int>& active);
+    void invalidate_graph()
+    {
+        if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+    }
+    ~GtfBatch()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        invalidate_graph();
+        sys.release();
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// store src_m into slot slot[m] of a set of padded vectors
+__global__ void bstore_slot_kernel(const long* __restrict__ poff, const int* __restrict__ pld,
+                                   const int* __restrict__ skip, const int* __restrict__ slot,
+                                   const double* __restrict__ src, double* __restrict__ vecs, long stride)
+{
+    const int m = blockIdx.y;
+    if (skip[m]) return;
+    const long nn = (long)pld[m] * pld[m];
+    double* dst = vecs + (long)slot[m] * stride + poff[m];
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < nn; k += (long)gridDim.x * blockDim.x)
+        dst[k] = src[poff[m] + k];
+}
+
+// X = U diag(lambda^-1/2) U^T into a padded block (pfock/pfock.c:1823-1861);
+// W holds the eigenvectors as rows (row-major n x n)
+__global__ void binv_sqrt_kernel(const double* W, const double* lam, int n, int ld, double* X)
+{
+    const int i = blockIdx.y * 16 + threadIdx.y, j = blockIdx.x * 16 + threadIdx.x;
+    if (i >= ld || j >= ld) return;
+    double s = 0.0;
+    if (i < n && j < n)
+        for (int k = 0; k < n; k++) s += W[(long)k * n + i] * W[(long)k * n + j] / sqrt(lam[k]);
+    X[(long)i * ld + j] = s;
+}
+
+int GtfBatch::enqueue_fock()
+{
+    if (gtf_fill(stream, d_J, total, 0.0) || gtf_fill(stream, d_K, total, 0.0)) return -1;
+    if (sys.jk(d_D, d_J, d_K)) return -1;
+    if (sys.symmetrize(d_J) || sys.symmetrize(d_K)) return -1;
+    return gtf_add(stream, d_J, d_K, d_G, total);
+}
+
+int GtfBatch::fock()
+{
+    B_CHECK(hipEventRecord(ev0, stream));
+    if (!use_graph || sys.timing) {
+        if (enqueue_fock()) return -1;
+    } else {
+        if (!gexec) {
+            // the class streams and events are created by the first build
+            if (!sys.cstream[0]) {
+                if (enqueue_fock()) return -1;
+                B_CHECK(hipStreamSynchronize(stream));
+            }
+            B_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue_fock();
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(stream, &g);
+            if (rc || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                GTF_PRINTF("graph capture of the batched build failed\n");
+                return -1;
+            }
+            graph = g;
+            B_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        }
+        B_CHECK(hipGraphLaunch(gexec, stream));
+    }
+    B_CHECK(hipEventRecord(ev1, stream));
+    B_CHECK(hipEventSynchronize(ev1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    last_ms = ms;
+    if (sys.collect_times()) return -1;
+    nbuilds++;
+    return 0;
+}
+
+int GtfBatch::scf_setup()
+{
+    if (scf_ready) return 0;
+    GtfSystem& s = sys;
+    d_H = s.dalloc<double>(total);
+    d_S = s.dalloc<double>(total);
+    d_F = s.dalloc<double>(total);
+    double* T = s.dalloc<double>(total);
+    double* V = s.dalloc<double>(total);
+    double** pads[] = {&p_X, &p_S, &p_F, &p_D, &p_D2, &p_D3, &p_W, &p_W2};
+    for (double** p : pads) *p = s.dalloc<double>(ptotal);
+    evecs = s.dalloc<double>((size_t)ptotal * B_MAX_DIIS);
+    Fvecs = s.dalloc<double>((size_t)ptotal * B_MAX_DIIS);
+    d_stats = s.dalloc<double>(3 * nmol);
+    d_dots = s.dalloc<double>((size_t)BMAX_DOTS * nmol);
+    d_coef = s.dalloc<double>((size_t)BMAX_DOTS * nmol);
+    d_energy = s.dalloc<double>(nmol);
+    d_iters = s.dalloc<int>(nmol);
+    d_skip = s.dalloc<int>(nmol);
+    d_nvec = s.dalloc<int>(nmol);
+    d_slot = s.dalloc<int>(nmol);
+    if (!d_H || !d_S || !d_F || !T || !V || !evecs || !Fvecs || !d_stats || !d_dots || !d_coef || !d_energy ||
+        !d_iters || !d_skip || !d_nvec || !d_slot)
+        return -2;
+    for (double** p : pads)
+        if (!*p) return -2;
+    for (double** p : pads)
+        if (gtf_fill(stream, *p, ptotal, 0.0)) return -1;
+    if (gtf_fill(stream, evecs, ptotal * B_MAX_DIIS, 0.0) || gtf_fill(stream, Fvecs, ptotal * B_MAX_DIIS, 0.0))
+        return -1;
+    // one-electron matrices per molecule over its slice of the shell arrays
+    for (int m = 0; m < nmol; m++) {
+        const int s0 = s.mol_shell_off[m], a0 = atom_off[m];
+        BasisView bv{s.mol_nshells[m], n[m], bases[m]->natoms, s.d_L + s0, s.d_nprim + s0, s.d_poff + s0,
+                     s.d_fstart + s0, s.d_atom + s0, s.d_alpha, s.d_coef, s.d_center + 3 * s0,
+                     s.d_atom_xyz + 3 * a0, s.d_atom_Z + a0};
+        if (gtf_oneel(stream, bv, s.d_boys, n[m], d_S + off[m], T + off[m], V + off[m])) return -1;
+    }
+    if (gtf_add(stream, T, V, d_H, total)) return -1;
+    // X = S^-1/2 per molecule
+    double* W = s.dalloc<double>((size_t)maxn * maxn);
+    double* lam = s.dalloc<double>(maxn);
+    double* work = s.dalloc<double>(maxn);
+    int* info = s.dalloc<int>(1);
+    if (!W || !lam || !work || !info) return -2;
+    rocblas_handle h;
+    if (rocblas_create_handle(&h) != rocblas_status_success) return -1;
+    rocblas_set_stream(h, stream);
+    int bad = 0;
+    for (int m = 0; m < nmol && !bad; m++) {
+        const int nm = n[m];
+        if (hipMemcpyAsync(W, d_S + off[m], sizeof(double) * nm * nm, hipMemcpyDeviceToDevice, stream) != hipSuccess) {
+            bad = 1;
+            break;
+        }
+        if (rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, nm, W, nm, lam, work, info) !=
+            rocblas_status_success) {
+            bad = 2;
+            break;
+        }
+        int hinfo = 0;
+        if (hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess || hinfo != 0) {
+            bad = 3;
+            break;
+        }
+        dim3 g((pld[m] + 15) / 16, (pld[m] + 15) / 16), b(16, 16);
+        binv_sqrt_kernel<<<g, b, 0, stream>>>(W, lam, nm, pld[m], p_X + poff[m]);
+        if (hipGetLastError() != hipSuccess) bad = 4;
+    }
+    rocblas_destroy_handle(h);
+    if (bad) {
+        GTF_PRINTF("S^-1/2 failed (%d)\n", bad);
+        return -1;
+    }
+    const BatchLayout L = layout();
+    if (gtf_bpad(stream, L, nullptr, d_S, p_S)) return -1;
+    B_CHECK(hipStreamSynchronize(stream));
+    s.dfree(T);
+    s.dfree(V);
+    s.dfree(W);
+    s.dfree(lam);
+    s.dfree(work);
+    s.dfree(info);
+    scf_ready = true;
+    return 0;
+}
+
+static void binverse(int m, const double* A, int lda, double* inv)
+{
+    double a[(B_MAX_DIIS + 1) * (B_MAX_DIIS + 1)];
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < m; j++) {
+            a[i * m + j] = A[i * lda + j];
+            inv[i * m + j] = (i == j);
+        }
+    for (int col = 0; col < m; col++) {
+        int piv = col;
+        for (int r = col + 1; r < m; r++)
+            if (fabs(a[r * m + col]) > fabs(a[piv * m + col])) piv = r;
+        if (piv != col)
+            for (int k = 0; k < m; k++) {
+                std::swap(a[col * m + k], a[piv * m + k]);
+                std::swap(inv[col * m + k], inv[piv * m + k]);
+            }
+        const double dg = a[col * m + col];
+        for (int k = 0; k < m; k++) {
+            a[col * m + k] /= dg;
+            inv[col * m + k] /= dg;
+        }
+        for (int r = 0; r < m; r++) {
+            if (r == col) continue;
+            const double f = a[r * m + col];
+            if (f == 0.0) continue;
+            for (int k = 0; k < m; k++) {
+                a[r * m + k] -= f * a[col * m + k];
+                inv[r * m + k] -= f * inv[col * m + k];
+            }
+        }
+    }
+}
+
+// compute_diis (pscf/purif.c:491-641) for the active molecules: F (packed,
+// AO) and D (packed) in, the orthogonal-basis extrapolated F in p_F
+int GtfBatch::diis(int iter, const std::vector<int>& active)
+{
+    const BatchLayout L = layout();
+    const int LB = B_MAX_DIIS + 1;
+    const int* skip = d_skip;
+    if (gtf_bpad(stream, L, skip, d_F, p_W2) || gtf_bpad(stream, L, skip, d_D, p_D)) return -1;
+    std::vector<int> slot(nmol, 0), nvec(nmol, 0);
+    if (iter > 1) {
+        for (int m = 0; m < nmol; m++) {
+            if (!active[m]) continue;
+            slot[m] = len_diis[m] < B_MAX_DIIS ? len_diis[m]++ : bmax_id[m];
+            nvec[m] = len_diis[m];
+        }
+        B_CHECK(hipMemcpyAsync(d_slot, slot.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, stream));
+        B_CHECK(hipMemcpyAsync(d_nvec, nvec.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, stream));
+        // e = X (F D S - S D F) X with the A * B^T convention of pdgemm3D_2
+        if (gtf_bgemm(stream, L, skip, p_W2, p_D, p_W, 1) || gtf_bgemm(stream, L, skip, p_W, p_S, p_D2, 1) ||
+            gtf_bantisym(stream, L, skip, p_D2, p_D3) || gtf_bgemm(stream, L, skip, p_X, p_D3, p_W, 1) ||
+            gtf_bgemm(stream, L, skip, p_W, p_X, p_D2, 1))
+            return -1;
+        bstore_slot_kernel<<<dim3(256, nmol), 256, 0, stream>>>(d_poff, d_pld, skip, d_slot, p_D2, evecs, ptotal);
+        B_CHECK(hipGetLastError());
+        if (gtf_bmultidot(stream, L, skip, d_nvec, p_D2, evecs, ptotal, d_dots)) return -1;
+        std::vector<double> dots((size_t)BMAX_DOTS * nmol);
+        B_CHECK(hipMemcpyAsync(dots.data(), d_dots, sizeof(double) * dots.size(), hipMemcpyDeviceToHost, stream));
+        B_CHECK(hipStreamSynchronize(stream));
+        for (int m = 0; m < nmol; m++) {
+            if (!active[m]) continue;
+            double* bm = &b_mat[(size_t)m * LB * LB];
+            const int cur = slot[m];
+            for (int i = 0; i < len_diis[m]; i++) {
+                bm[cur * LB + i] = dots[(size_t)m * BMAX_DOTS + i];
+                bm[i * LB + cur] = dots[(size_t)m * BMAX_DOTS + i];
+            }
+            bmax[m] = -DBL_MAX;
+            for (int i = 0; i < len_diis[m]; i++)
+                if (bmax[m] < bm[i * LB + i]) {
+                    bmax[m] = bm[i * LB + i];
+                    bmax_id[m] = i;
+                }
+        }
+    }
+    // F_orth = X F X
+    if (gtf_bgemm(stream, L, skip, p_X, p_W2, p_W, 1) || gtf_bgemm(stream, L, skip, p_W, p_X, p_F, 1)) return -1;
+    if (iter > 1) {
+        bstore_slot_kernel<<<dim3(256, nmol), 256, 0, stream>>>(d_poff, d_pld, skip, d_slot, p_F, Fvecs, ptotal);
+        B_CHECK(hipGetLastError());
+        std::vector<double> coef((size_t)BMAX_DOTS * nmol, 0.0);
+        for (int m = 0; m < nmol; m++) {
+            if (!active[m]) continue;
+            const int sz = len_diis[m] + 1;
+            double inv[(B_MAX_DIIS + 1) * (B_MAX_DIIS + 1)];
+            binverse(sz, &b_mat[(size_t)m * LB * LB], LB, inv);
+            for (int i = 0; i < len_diis[m]; i++) coef[(size_t)m * BMAX_DOTS + i] = -inv[i * sz + sz - 1];
+        }
+        B_CHECK(hipMemcpyAsync(d_coef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, stream));
+        if (gtf_blincomb(stream, L, skip, d_nvec, d_coef, Fvecs, ptotal, p_F)) return -1;
+    }
+    return 0;
+}
+
+// compute_purification (pscf/purif.c:257-488) for the active molecules:
+// p_F (orthogonal basis) in, the AO density in d_D (packed)
+int GtfBatch::purify(const std::vector<int>& active)
+{
+    const BatchLayout L = layout();
+    const int* skip = d_skip;
+    if (gtf_bpurif_init(stream, L, d_nocc, skip, p_F, p_D)) return -1;
+    if (gtf_fill_int(stream, d_iters, nmol, -1)) return -1;
+    std::vector<int> it_h(nmol, -1);
+    int it = 0;
+    for (it = 0; it < 200; it++) {
+        // molecules already converged skip their GEMMs through d_iters >= 0
+        if (gtf_bgemm(stream, L, d_iters, p_D, p_D, p_D2, 0) || gtf_bgemm(stream, L, d_iters, p_D, p_D2, p_D3, 0))
+            return -1;
+        if (gtf_bpurif_stats(stream, L, skip, p_D, p_D2, p_D3, d_iters, d_stats, it, 1e-11)) return -1;
+        if (gtf_bpurif_update(stream, L, skip, d_iters, d_stats, p_D, p_D2, p_D3, it)) return -1;
+        if (it % 2 == 1) {
+            B_CHECK(hipMemcpyAsync(it_h.data(), d_iters, sizeof(int) * nmol, hipMemcpyDeviceToHost, stream));
+            B_CHECK(hipStreamSynchronize(stream));
+            bool all = true;
+            for (int m = 0; m < nmol; m++)
+                if (active[m] && it_h[m] < 0) all = false;
+            if (all) break;
+        }
+    }
+    B_CHECK(hipMemcpyAsync(it_h.data(), d_iters, sizeof(int) * nmol, hipMemcpyDeviceToHost, stream));
+    B_CHECK(hipStreamSynchronize(stream));
+    for (int m = 0; m < nmol; m++)
+        if (active[m]) purif_iters[m] = it_h[m] < 0 ? 200 : it_h[m];
+    // D_AO = X D X
+    if (gtf_bgemm(stream, L, skip, p_X, p_D, p_W, 1) || gtf_bgemm(stream, L, skip, p_W, p_X, p_D2, 1)) return -1;
+    return gtf_bunpad(stream, L, skip, p_D2, d_D);
+}
+
+extern "C" {
+
+int gtf_batch_create(BasisSet_t* basis, int nmol, double tolscr, gtf_batch_t* out)
+{
+    if (!basis || nmol <= 0 || !out || tolscr < 0.0) return -3;
+    *out = nullptr;
+    const double t0 = bwtime();
+    GtfBatch* b = new (std::nothrow) GtfBatch();
+    if (!b) return -2;
+    b->nmol = nmol;
+    int natoms = 0;
+    for (int m = 0; m < nmol; m++) {
+        if (!basis[m] || basis[m]->nshells == 0) { delete b; return -3; }
+        b->bases.push_back(basis[m]);
+        const int nm = basis[m]->nbf, pl = ((nm + 63) / 64) * 64;
+        b->n.push_back(nm);
+        b->pld.push_back(pl);
+        b->nocc.push_back(CInt_getNumOccOrb(basis[m]));
+        b->atom_off.push_back(natoms);
+        natoms += basis[m]->natoms;
+        b->off.push_back(b->total);
+        b->poff.push_back(b->ptotal);
+        b->total += (long)nm * nm;
+        b->ptotal += (long)pl * pl;
+        b->maxn = std::max(b->maxn, nm);
+        b->maxpld = std::max(b->maxpld, pl);
+        b->enuc.push_back(CInt_getNucEnergy(basis[m]));
+    }
+    b->len_diis.assign(nmol, 0);
+    b->bmax_id.assign(nmol, -1);
+    b->bmax.assign(nmol, DBL_MIN);
+    b->purif_iters.assign(nmol, 0);
+    if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
+        hipEventCreate(&b->ev1) != hipSuccess) {
+        delete b;
+        return -1;
+    }
+    int rc = b->sys.init(b->bases, b->stream);
+    if (rc == 0) rc = b->sys.screen(tolscr);
+    if (rc) { delete b; return rc; }
+    GtfSystem& s = b->sys;
+    b->d_off = s.dalloc<long>(nmol);
+    b->d_poff = s.dalloc<long>(nmol);
+    b->d_n = s.dalloc<int>(nmol);
+    b->d_pld = s.dalloc<int>(nmol);
+    b->d_nocc = s.dalloc<int>(nmol);
+    b->d_D = s.dalloc<double>(b->total);
+    b->d_J = s.dalloc<double>(b->total);
+    b->d_K = s.dalloc<double>(b->total);
+    b->d_G = s.dalloc<double>(b->total);
+    if (!b->d_off || !b->d_poff || !b->d_n || !b->d_pld || !b->d_nocc || !b->d_D || !b->d_J || !b->d_K || !b->d_G) {
+        delete b;
+        return -2;
+    }
+    if (hipMemcpyAsync(b->d_off, b->off.data(), sizeof(long) * nmol, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(b->d_poff, b->poff.data(), sizeof(long) * nmol, hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(b->d_n, b->n.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(b->d_pld, b->pld.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(b->d_nocc, b->nocc.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess ||
+        gtf_fill(b->stream, b->d_D, b->total, 0.0) || hipStreamSynchronize(b->stream) != hipSuccess) {
+        delete b;
+        return -1;
+    }
+    b->create_s = bwtime() - t0;
+    *out = b;
+    return 0;
+}
+
+void gtf_batch_destroy(gtf_batch_t b) { delete b; }
+
+int gtf_batch_layout(gtf_batch_t b, int* nbf, long* offset)
+{
+    if (!b) return -3;
+    for (int m = 0; m < b->nmol; m++) {
+        if (nbf) nbf[m] = b->n[m];
+        if (offset) offset[m] = b->off[m];
+    }
+    return b->nmol;
+}
+
+long gtf_batch_total(gtf_batch_t b) { return b ? b->total : -1; }
+
+int gtf_batch_set_precision(gtf_batch_t b, double tau)
+{
+    if (!b || tau < 0.0) return -3;
+    b->sys.tau = tau;
+    b->invalidate_graph();
+    return b->sys.build_tiles();
+}
+
+int gtf_batch_use_graph(gtf_batch_t b, int on)
+{
+    if (!b) return -3;
+    b->use_graph = on != 0;
+    return 0;
+}
+
+int gtf_batch_put_density(gtf_batch_t b, int m, const double* D)
+{
+    if (!b || !D || m >= b->nmol) return -3;
+    const long o = m < 0 ? 0 : b->off[m];
+    const long cnt = m < 0 ? b->total : (long)b->n[m] * b->n[m];
+    if (hipMemcpyAsync(b->d_D + o, D, sizeof(double) * cnt, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+        return -1;
+    return 0;
+}
+
+int gtf_batch_compute_fock(gtf_batch_t b)
+{
+    if (!b) return -3;
+    return b->fock();
+}
+
+int gtf_batch_get_fock(gtf_batch_t b, int m, double* G)
+{
+    if (!b || !G || m >= b->nmol) return -3;
+    const long o = m < 0 ? 0 : b->off[m];
+    const long cnt = m < 0 ? b->total : (long)b->n[m] * b->n[m];
+    if (hipMemcpyAsync(G, b->d_G + o, sizeof(double) * cnt, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+        return -1;
+    return 0;
+}
+
+int gtf_batch_device_ptrs(gtf_batch_t b, double** D, double** G)
+{
+    if (!b) return -3;
+    if (D) *D = b->d_D;
+    if (G) *G = b->d_G;
+    return 0;
+}
+
+void* gtf_batch_stream(gtf_batch_t b) { return b ? (void*)b->stream : nullptr; }
+
+int gtf_batch_stats(gtf_batch_t b, double* out)
+{
+    if (!b || !out) return -3;
+    double nq = 0, npq = 0, nq32 = 0, npq32 = 0;
+    for (int c = 0; c < GTF_NCLASSES; c++) {
+        nq += (double)b->sys.nq[c];
+        npq += b->sys.nprimq[c];
+        nq32 += (double)b->sys.nq_lo[c];
+        npq32 += b->sys.nprimq_lo[c];
+    }
+    out[0] = nq;
+    out[1] = npq;
+    out[2] = nq32;
+    out[3] = npq32;
+    out[4] = b->last_ms;
+    out[5] = b->gexec ? 1.0 : 0.0;
+    out[6] = (double)b->sys.dev_bytes;
+    out[7] = b->create_s;
+    out[8] = b->time_fock;
+    out[9] = b->time_diis;
+    out[10] = b->time_purif;
+    return 11;
+}
+
+int gtf_batch_class_work(gtf_batch_t b, long* nquartets, double* nprimquartets, long* nq_fp32, double* npq_fp32)
+{
+    if (!b) return -3;
+    for (int c = 0; c < GTF_NCLASSES; c++) {
+        if (nquartets) nquartets[c] = b->sys.nq[c];
+        if (nprimquartets) nprimquartets[c] = b->sys.nprimq[c];
+        if (nq_fp32) nq_fp32[c] = b->sys.nq_lo[c];
+        if (npq_fp32) npq_fp32[c] = b->sys.nprimq_lo[c];
+    }
+    return GTF_NCLASSES;
+}
+
+int gtf_batch_scf(gtf_batch_t b, int max_iters, double ediff, double* energy, int* iters, double* history)
+{
+    if (!b || max_iters <= 0 || !energy) return -3;
+    if (b->scf_setup()) return -1;
+    const int nmol = b->nmol;
+    const int LB = B_MAX_DIIS + 1;
+    b->len_diis.assign(nmol, 0);
+    b->bmax_id.assign(nmol, -1);
+    b->bmax.assign(nmol, DBL_MIN);
+    b->b_mat.assign((size_t)nmol * LB * LB, 0.0);
+    for (int m = 0; m < nmol; m++)
+        for (int i = 0; i <= B_MAX_DIIS; i++)
+            for (int j = 0; j <= B_MAX_DIIS; j++) b->b_mat[(size_t)m * LB * LB + i * LB + j] = (i != j) ? -1.0 : 0.0;
+    b->time_fock = b->time_diis = b->time_purif = 0.0;
+    if (gtf_fill(b->stream, b->d_D, b->total, 0.0)) return -1;
+    std::vector<int> active(nmol, 1), skip(nmol, 0), nit(nmol, max_iters);
+    std::vector<double> e0(nmol, -1.0), e(nmol, 0.0);
+    const BatchLayout L = b->layout();
+    for (int it = 0; it < max_iters; it++) {
+        double t0 = bwtime();
+        if (b->fock()) return -1;
+        if (gtf_bfock_energy(b->stream, L, b->d_H, b->d_G, b->d_D, b->d_F, b->d_energy)) return -1;
+        B_CHECK(hipMemcpyAsync(e.data(), b->d_energy, sizeof(double) * nmol, hipMemcpyDeviceToHost, b->stream));
+        B_CHECK(hipStreamSynchronize(b->stream));
+        b->time_fock += bwtime() - t0;
+        int nactive = 0;
+        for (int m = 0; m < nmol; m++) {
+            if (!active[m]) continue;
+            energy[m] = e[m] + b->enuc[m];
+            if (history) history[(size_t)m * max_iters + it] = energy[m];
+            if (it > 0 && fabs(e[m] - e0[m]) < ediff) {
+                active[m] = 0;
+                nit[m] = it + 1;
+            } else {
+                e0[m] = e[m];
+                nactive++;
+            }
+        }
+        if (nactive == 0 || it == max_iters - 1) break;
+        for (int m = 0; m < nmol; m++) skip[m] = !active[m];
+        B_CHECK(hipMemcpyAsync(b->d_skip, skip.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, b->stream));
+        t0 = bwtime();
+        if (b->diis(it, active)) return -1;
+        B_CHECK(hipStreamSynchronize(b->stream));
+        const double t1 = bwtime();
+        b->time_diis += t1 - t0;
+        if (b->purify(active)) return -1;
+        B_CHECK(hipStreamSynchronize(b->stream));
+        b->time_purif += bwtime() - t1;
+    }
+    if (iters)
+        for (int m = 0; m < nmol; m++) iters[m] = nit[m];
+    return 0;
+}
+
+int gtf_batch_get_density(gtf_batch_t b, int m, double* D)
+{
+    if (!b || !D || m >= b->nmol) return -3;
+    const long o = m < 0 ? 0 : b->off[m];
+    const long cnt = m < 0 ? b->total : (long)b->n[m] * b->n[m];
+    if (hipMemcpyAsync(D, b->d_D + o, sizeof(double) * cnt, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+        return -1;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,36 @@
+// batch_linalg.h -- launchers of batch_linalg.hip (dense SCF steps over a
+// batch of molecules).
+#pragma once
+#include <algorithm>
+
+#include "gtf_internal.h"
+
+// per-molecule DIIS dot products / coefficients (MAX_DIIS + 1 rounded up)
+#define BMAX_DOTS 12
+
+// device arrays describing the two per-molecule layouts
+struct BatchLayout {
+    int nmol;
+    int maxn, maxpld;
+    const long* off;   // packed block offsets (n x n)
+    const int* n;
+    const long* poff;  // padded block offsets (pld x pld)
+    const int* pld;
+};
+
+int gtf_bgemm(hipStream_t st, const BatchLayout& L, const int* skip, const double* A, const double* B, double* C,
+              int transB);
+int gtf_bpad(hipStream_t st, const BatchLayout& L, const int* skip, const double* packed, double* padded);
+int gtf_bunpad(hipStream_t st, const BatchLayout& L, const int* skip, const double* padded, double* packed);
+int gtf_bpurif_init(hipStream_t st, const BatchLayout& L, const int* nocc, const int* skip, const double* F, double* D);
+int gtf_bpurif_stats(hipStream_t st, const BatchLayout& L, const int* skip, const double* D, const double* D2,
+                     const double* D3, int* iters, double* stats, int it, double tol);
+int gtf_bpurif_update(hipStream_t st, const BatchLayout& L, const int* skip, const int* iters, const double* stats,
+                      double* D, const double* D2, const double* D3, int it);
+int gtf_bantisym(hipStream_t st, const BatchLayout& L, const int* skip, const double* W, double* E);
+int gtf_bmultidot(hipStream_t st, const BatchLayout& L, const int* skip, const int* nvec, const double* x,
+                  const double* vecs, long stride, double* dots);
+int gtf_blincomb(hipStream_t st, const BatchLayout& L, const int* skip, const int* nvec, const double* coef,
+                 const double* vecs, long stride, double* F);
+int gtf_bfock_energy(hipStream_t st, const BatchLayout& L, const double* H, const double* G, const double* D,
+                     double* F, double* energy);

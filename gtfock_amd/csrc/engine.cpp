@@ -35,63 +35,114 @@ static int upload(T* d, const std::vector<T>& h, hipStream_t st)
     return 0;
 }
 
-int GtfSystem::init(const BasisSet* b, hipStream_t st)
+int GtfSystem::init(const std::vector<const BasisSet*>& mols, hipStream_t st)
 {
     stream = st;
-    nbf = b->nbf;
-    nshells = b->nshells;
-    natoms = b->natoms;
-    std::vector<double> center(3 * nshells), atomZ(natoms);
-    for (int s = 0; s < nshells; s++)
-        for (int d = 0; d < 3; d++) center[3 * s + d] = b->xyz[3 * b->atom[s] + d];
-    for (int a = 0; a < natoms; a++) atomZ[a] = b->Z[a];
+    nmol = (int)mols.size();
+    if (nmol == 0) return -3;
+    nbf = nshells = natoms = 0;
+    mol_nbf.assign(nmol, 0);
+    mol_shell_off.assign(nmol, 0);
+    mol_nshells.assign(nmol, 0);
+    mol_pair_off.assign(nmol, 0);
+    mol_npairs.assign(nmol, 0);
+    mol_mat_off.assign(nmol, 0);
+    mat_total = 0;
+    npairs = 0;
+    std::vector<int> L, nprim, poff, fstart, atom, mld;
+    std::vector<long> mbase;
+    std::vector<double> alpha, coef, center, atom_xyz, atomZ;
+    h_shell_mol.clear();
+    for (int m = 0; m < nmol; m++) {
+        const BasisSet* b = mols[m];
+        mol_nbf[m] = b->nbf;
+        mol_shell_off[m] = nshells;
+        mol_nshells[m] = b->nshells;
+        mol_pair_off[m] = npairs;
+        mol_npairs[m] = b->nshells * (b->nshells + 1) / 2;
+        mol_mat_off[m] = mat_total;
+        for (int s = 0; s < b->nshells; s++) {
+            L.push_back(b->L[s]);
+            nprim.push_back(b->nprim[s]);
+            poff.push_back((int)alpha.size() + b->poff[s]);
+            fstart.push_back(b->fstart[s]);
+            atom.push_back(natoms + b->atom[s]);
+            mbase.push_back(mat_total);
+            mld.push_back(b->nbf);
+            h_shell_mol.push_back(m);
+            for (int d = 0; d < 3; d++) center.push_back(b->xyz[3 * b->atom[s] + d]);
+        }
+        alpha.insert(alpha.end(), b->alpha.begin(), b->alpha.end());
+        coef.insert(coef.end(), b->coef.begin(), b->coef.end());
+        atom_xyz.insert(atom_xyz.end(), b->xyz.begin(), b->xyz.end());
+        for (int a = 0; a < b->natoms; a++) atomZ.push_back(b->Z[a]);
+        nbf = std::max(nbf, b->nbf);
+        nshells += b->nshells;
+        natoms += b->natoms;
+        npairs += mol_npairs[m];
+        mat_total += (long)b->nbf * b->nbf;
+    }
+    fstart.push_back(mols[nmol - 1]->nbf);
+    std::vector<int> moln(mol_nbf);
     d_L = dalloc<int>(nshells);
     d_nprim = dalloc<int>(nshells);
     d_poff = dalloc<int>(nshells);
     d_fstart = dalloc<int>(nshells + 1);
     d_atom = dalloc<int>(nshells);
-    d_alpha = dalloc<double>(b->alpha.size());
-    d_coef = dalloc<double>(b->coef.size());
+    d_mbase = dalloc<long>(nshells);
+    d_mld = dalloc<int>(nshells);
+    d_mol_off = dalloc<long>(nmol);
+    d_mol_n = dalloc<int>(nmol);
+    d_alpha = dalloc<double>(alpha.size());
+    d_coef = dalloc<double>(coef.size());
     d_center = dalloc<double>(3 * nshells);
     d_atom_xyz = dalloc<double>(3 * natoms);
     d_atom_Z = dalloc<double>(natoms);
     d_boys = dalloc<double>((size_t)BOYS_NGRID * BOYS_NCOL);
-    if (!d_L || !d_nprim || !d_poff || !d_fstart || !d_atom || !d_alpha || !d_coef || !d_center || !d_atom_xyz ||
-        !d_atom_Z || !d_boys)
+    if (!d_L || !d_nprim || !d_poff || !d_fstart || !d_atom || !d_mbase || !d_mld || !d_mol_off || !d_mol_n ||
+        !d_alpha || !d_coef || !d_center || !d_atom_xyz || !d_atom_Z || !d_boys)
         return -2;
-    if (upload(d_L, b->L, st) || upload(d_nprim, b->nprim, st) || upload(d_poff, b->poff, st) ||
-        upload(d_fstart, b->fstart, st) || upload(d_atom, b->atom, st) || upload(d_alpha, b->alpha, st) ||
-        upload(d_coef, b->coef, st) || upload(d_center, center, st) || upload(d_atom_xyz, b->xyz, st) ||
-        upload(d_atom_Z, atomZ, st))
+    if (upload(d_L, L, st) || upload(d_nprim, nprim, st) || upload(d_poff, poff, st) ||
+        upload(d_fstart, fstart, st) || upload(d_atom, atom, st) || upload(d_mbase, mbase, st) ||
+        upload(d_mld, mld, st) || upload(d_mol_off, mol_mat_off, st) || upload(d_mol_n, moln, st) ||
+        upload(d_alpha, alpha, st) || upload(d_coef, coef, st) || upload(d_center, center, st) ||
+        upload(d_atom_xyz, atom_xyz, st) || upload(d_atom_Z, atomZ, st))
         return -1;
     std::vector<double> boys((size_t)BOYS_NGRID * BOYS_NCOL);
     gtf_boys_table(boys.data(), BOYS_NGRID, BOYS_NCOL, BOYS_DELTA);
     if (upload(d_boys, boys, st)) return -1;
 
-    // all unordered shell pairs (A >= B by index), canonical orientation by L
-    npairs = nshells * (nshells + 1) / 2;
+    // all unordered shell pairs (A >= B by local index) of every molecule,
+    // canonical orientation by L
     h_s1.resize(npairs);
     h_s2.resize(npairs);
     h_type.resize(npairs);
     h_pnprim.resize(npairs);
+    h_pair_mol.resize(npairs);
     std::vector<int> ppoff(npairs);
-    std::vector<long> peoff(npairs);
-    long np = 0, ne = 0;
-    for (int A = 0; A < nshells; A++)
-        for (int B = 0; B <= A; B++) {
-            const int id = A * (A + 1) / 2 + B;
-            int s1 = A, s2 = B;
-            if (b->L[B] > b->L[A]) std::swap(s1, s2);
-            const int la = b->L[s1], lb = b->L[s2];
-            h_s1[id] = s1;
-            h_s2[id] = s2;
-            h_type[id] = pair_type(la, lb);
-            h_pnprim[id] = b->nprim[s1] * b->nprim[s2];
-            ppoff[id] = (int)np;
-            peoff[id] = ne;
-            np += h_pnprim[id];
-            ne += (long)h_pnprim[id] * nsph(la) * nsph(lb) * nherm(la + lb);
-        }
+    std::vector<long> peoff(npairs), pzoff(npairs);
+    long np = 0, ne = 0, nz = 0;
+    for (int m = 0; m < nmol; m++) {
+        const int s0 = mol_shell_off[m];
+        for (int a = 0; a < mol_nshells[m]; a++)
+            for (int c = 0; c <= a; c++) {
+                const int id = mol_pair_off[m] + a * (a + 1) / 2 + c;
+                int s1 = s0 + a, s2 = s0 + c;
+                if (L[s2] > L[s1]) std::swap(s1, s2);
+                const int la = L[s1], lb = L[s2];
+                h_s1[id] = s1;
+                h_s2[id] = s2;
+                h_type[id] = pair_type(la, lb);
+                h_pnprim[id] = nprim[s1] * nprim[s2];
+                h_pair_mol[id] = m;
+                ppoff[id] = (int)np;
+                peoff[id] = ne;
+                np += h_pnprim[id];
+                ne += (long)h_pnprim[id] * nsph(la) * nsph(lb) * nherm(la + lb);
+                pzoff[id] = nz;
+                nz += (long)h_pnprim[id] * herm_nnz(la, lb);
+            }
+    }
     nprimpairs = np;
     nE = ne;
     d_s1 = dalloc<int>(npairs);
@@ -101,14 +152,18 @@ int GtfSystem::init(const BasisSet* b, hipStream_t st)
     d_peoff = dalloc<long>(npairs);
     d_prim = dalloc<double4>(np);
     d_E = dalloc<double>(ne);
+    d_pzoff = dalloc<long>(npairs);
+    d_Ez = dalloc<double>(nz);
     d_counters = dalloc<int>(2 * GTF_NCLASSES);
-    if (!d_s1 || !d_s2 || !d_pnprim || !d_ppoff || !d_peoff || !d_prim || !d_E || !d_counters) return -2;
+    if (!d_s1 || !d_s2 || !d_pnprim || !d_ppoff || !d_peoff || !d_prim || !d_E || !d_counters || !d_pzoff || !d_Ez)
+        return -2;
     if (upload(d_s1, h_s1, st) || upload(d_s2, h_s2, st) || upload(d_pnprim, h_pnprim, st) ||
-        upload(d_ppoff, ppoff, st) || upload(d_peoff, peoff, st))
+        upload(d_ppoff, ppoff, st) || upload(d_peoff, peoff, st) || upload(d_pzoff, pzoff, st))
         return -1;
     BasisView bv{nshells, nbf, natoms, d_L, d_nprim, d_poff, d_fstart, d_atom,
                  d_alpha, d_coef, d_center, d_atom_xyz, d_atom_Z};
-    if (gtf_build_pairs(st, bv, npairs, d_s1, d_s2, d_ppoff, d_peoff, d_prim, d_E, d_pnprim, prim_eps)) return -1;
+    if (gtf_build_pairs(st, bv, npairs, d_s1, d_s2, d_ppoff, d_peoff, d_prim, d_E, d_pzoff, d_Ez, d_pnprim, prim_eps))
+        return -1;
     // surviving primitive pairs per shell pair
     SYS_CHECK(hipMemcpyAsync(h_pnprim.data(), d_pnprim, sizeof(int) * npairs, hipMemcpyDeviceToHost, st));
     SYS_CHECK(hipStreamSynchronize(st));
@@ -117,9 +172,18 @@ int GtfSystem::init(const BasisSet* b, hipStream_t st)
     return 0;
 }
 
+void GtfSystem::dfree(void* p)
+{
+    if (!p) return;
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it == allocs.end()) return;
+    allocs.erase(it);
+    (void)hipFree(p);
+}
+
 PairView GtfSystem::pair_view() const
 {
-    return PairView{d_s1, d_s2, d_pnprim, d_ppoff, d_peoff, d_prim, d_E};
+    return PairView{d_s1, d_s2, d_pnprim, d_ppoff, d_peoff, d_prim, d_E, d_pzoff, d_Ez};
 }
 
 static int launch(eri_kernel_t fn, EriArgs& a, int blocks, hipStream_t st, size_t lds = 0)
@@ -132,6 +196,7 @@ static int launch(eri_kernel_t fn, EriArgs& a, int blocks, hipStream_t st, size_
 int GtfSystem::screen(double tolscr)
 {
     if (const char* s = getenv("GTF_JK_TILES")) use_tiles = atoi(s) != 0;
+    if (const char* s = getenv("GTF_JK_SERIAL")) serial = atoi(s) != 0;
     {
         int dev = 0, lds = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
@@ -142,14 +207,13 @@ int GtfSystem::screen(double tolscr)
     tol2 = tolscr * tolscr;
     const PairView pv = pair_view();
     unsigned long long* d_vbits = dalloc<unsigned long long>(npairs);
-    if (!d_vbits) return -2;
+    int2* d_q = dalloc<int2>(npairs);
+    int* d_ids = dalloc<int>(npairs);
+    if (!d_vbits || !d_q || !d_ids) return -2;
     SYS_CHECK(hipMemsetAsync(d_vbits, 0, sizeof(unsigned long long) * npairs, stream));
     // Schwarz values: (MN|MN) diagonal maxima, one launch per pair type
     std::vector<std::vector<int>> bytype(GTF_NPAIRTYPES);
     for (int id = 0; id < npairs; id++) bytype[h_type[id]].push_back(id);
-    int2* d_q = dalloc<int2>(npairs);
-    int* d_ids = dalloc<int>(npairs);
-    if (!d_q || !d_ids) return -2;
     {
         int off = 0;
         for (int t = 0; t < GTF_NPAIRTYPES; t++) {
@@ -172,52 +236,77 @@ int GtfSystem::screen(double tolscr)
     std::vector<unsigned long long> vb(npairs);
     SYS_CHECK(hipMemcpyAsync(vb.data(), d_vbits, sizeof(unsigned long long) * npairs, hipMemcpyDeviceToHost, stream));
     SYS_CHECK(hipStreamSynchronize(stream));
+    dfree(d_vbits);
+    dfree(d_q);
+    dfree(d_ids);
     h_v.resize(npairs);
-    vmax = 0.0;
+    mol_vmax.assign(nmol, 0.0);
     for (int id = 0; id < npairs; id++) {
         double v;
         memcpy(&v, &vb[id], sizeof v);
         h_v[id] = v;
-        vmax = std::max(vmax, v);
+        mol_vmax[h_pair_mol[id]] = std::max(mol_vmax[h_pair_mol[id]], v);
     }
-    // significant pairs: v > tol^2 / vmax (pfock/screening.c:111,129)
-    const double eta = tol2 / vmax;
+    vmax = mol_vmax[0];
+    // significant pairs of molecule m: v > tol^2 / vmax_m (pfock/screening.c:111,129);
+    // one segment per molecule in each type list
+    std::vector<double> eta(nmol);
+    for (int m = 0; m < nmol; m++) eta[m] = mol_vmax[m] > 0.0 ? tol2 / mol_vmax[m] : HUGE_VAL;
     for (int t = 0; t < GTF_NPAIRTYPES; t++) {
         std::vector<int>& s = h_sig[t];
         s.clear();
-        for (int id : bytype[t])
-            if (h_v[id] > eta) s.push_back(id);
-        std::sort(s.begin(), s.end(), [&](int x, int y) { return h_v[x] != h_v[y] ? h_v[x] > h_v[y] : x < y; });
+        h_sigv[t].clear();
+        h_rank[t].clear();
+        h_sigmol[t].clear();
+        h_sorted[t].clear();
+        seg_start[t].assign(nmol, 0);
+        seg_len[t].assign(nmol, 0);
+        // bytype[t] is ordered by pair id, hence grouped by molecule
+        size_t i0 = 0;
+        for (int m = 0; m < nmol; m++) {
+            std::vector<int> seg;
+            while (i0 < bytype[t].size() && h_pair_mol[bytype[t][i0]] == m) {
+                const int id = bytype[t][i0++];
+                if (h_v[id] > eta[m]) seg.push_back(id);
+            }
+            std::sort(seg.begin(), seg.end(),
+                      [&](int x, int y) { return h_v[x] != h_v[y] ? h_v[x] > h_v[y] : x < y; });
+            // value rank, then each block re-sorted by primitive count (descending)
+            std::vector<int> order(seg.size());
+            std::iota(order.begin(), order.end(), 0);
+            for (size_t b0 = 0; b0 < seg.size(); b0 += GTF_KET_BLOCK) {
+                const size_t b1 = std::min(seg.size(), b0 + GTF_KET_BLOCK);
+                std::stable_sort(order.begin() + b0, order.begin() + b1,
+                                 [&](int x, int y) { return h_pnprim[seg[x]] > h_pnprim[seg[y]]; });
+            }
+            seg_start[t][m] = (int)s.size();
+            seg_len[t][m] = (int)seg.size();
+            for (size_t k = 0; k < seg.size(); k++) {
+                s.push_back(seg[order[k]]);
+                h_rank[t].push_back(order[k]);
+                h_sigv[t].push_back(h_v[seg[order[k]]]);
+                h_sigmol[t].push_back(m);
+                h_sorted[t].push_back(h_v[seg[k]]);
+            }
+        }
         nsig[t] = (int)s.size();
-        std::vector<double> sorted(s.size());
-        for (size_t k = 0; k < s.size(); k++) sorted[k] = h_v[s[k]];
-        // value rank, then each block re-sorted by primitive count (descending)
-        std::vector<int> order(s.size());
-        std::iota(order.begin(), order.end(), 0);
-        for (size_t b0 = 0; b0 < s.size(); b0 += GTF_KET_BLOCK) {
-            const size_t b1 = std::min(s.size(), b0 + GTF_KET_BLOCK);
-            std::stable_sort(order.begin() + b0, order.begin() + b1,
-                             [&](int x, int y) { return h_pnprim[s[x]] > h_pnprim[s[y]]; });
-        }
-        std::vector<int> perm(s.size());
-        h_sigv[t].resize(s.size());
-        h_rank[t].resize(s.size());
-        for (size_t k = 0; k < s.size(); k++) {
-            perm[k] = s[order[k]];
-            h_rank[t][k] = order[k];
-            h_sigv[t][k] = sorted[order[k]];
-        }
-        s = perm;
         d_sig[t] = dalloc<int>(s.size());
         d_sigv[t] = dalloc<double>(s.size());
         d_rank[t] = dalloc<int>(s.size());
+        d_sigmol[t] = dalloc<int>(s.size());
         d_sigv_sorted[t] = dalloc<double>(s.size());
-        if (!d_sig[t] || !d_sigv[t] || !d_rank[t] || !d_sigv_sorted[t]) return -2;
+        d_seg_start[t] = dalloc<int>(nmol);
+        d_seg_len[t] = dalloc<int>(nmol);
+        if (!d_sig[t] || !d_sigv[t] || !d_rank[t] || !d_sigmol[t] || !d_sigv_sorted[t] || !d_seg_start[t] ||
+            !d_seg_len[t])
+            return -2;
         if (upload(d_sig[t], s, stream) || upload(d_sigv[t], h_sigv[t], stream) ||
-            upload(d_rank[t], h_rank[t], stream) || upload(d_sigv_sorted[t], sorted, stream))
+            upload(d_rank[t], h_rank[t], stream) || upload(d_sigmol[t], h_sigmol[t], stream) ||
+            upload(d_sigv_sorted[t], h_sorted[t], stream) || upload(d_seg_start[t], seg_start[t], stream) ||
+            upload(d_seg_len[t], seg_len[t], stream))
             return -1;
     }
-    // quartet rows per class: bra type A >= ket type B
+    // quartet rows per class: bra type A >= ket type B, ket in the bra's molecule
     int maxn = 1;
     for (int t = 0; t < GTF_NPAIRTYPES; t++) maxn = std::max(maxn, nsig[t]);
     long* d_counts = dalloc<long>(maxn);
@@ -227,45 +316,82 @@ int GtfSystem::screen(double tolscr)
         int A, B;
         gtf_class_types(cls, A, B);
         nq[cls] = 0;
+        nslots[cls] = 0;
+        nprimq[cls] = 0.0;
+        h_exact[cls].assign(nsig[A], 0);
         d_rowoff[cls] = dalloc<long>(nsig[A] + 1);
         if (!d_rowoff[cls]) return -2;
         if (nsig[A] == 0 || nsig[B] == 0) continue;
-        if (gtf_count_rows(stream, nsig[A], d_sigv[A], d_rank[A], nsig[B], d_sigv_sorted[B], tol2, A == B,
-                           d_exact, d_counts, d_rowoff[cls]))
+        if (gtf_count_rows(stream, nsig[A], d_sigv[A], d_rank[A], d_sigmol[A], d_sigv_sorted[B], d_seg_start[B],
+                           d_seg_len[B], tol2, A == B, d_exact, d_counts, d_rowoff[cls]))
             return -1;
         std::vector<long> off(nsig[A] + 1), exact(nsig[A]);
         SYS_CHECK(hipMemcpyAsync(off.data(), d_rowoff[cls], sizeof(long) * off.size(), hipMemcpyDeviceToHost, stream));
         SYS_CHECK(hipMemcpyAsync(exact.data(), d_exact, sizeof(long) * exact.size(), hipMemcpyDeviceToHost, stream));
         SYS_CHECK(hipStreamSynchronize(stream));
         nslots[cls] = off[nsig[A]];
-        // exact quartet and primitive-quartet counts (value order)
-        std::vector<double> kp(nsig[B] + 1, 0.0);
-        {
-            std::vector<int> byrank(nsig[B]);
-            for (int j = 0; j < nsig[B]; j++) byrank[h_rank[B][j]] = h_sig[B][j];
-            for (int j = 0; j < nsig[B]; j++) kp[j + 1] = kp[j] + h_pnprim[byrank[j]];
-        }
+        // exact quartet and primitive-quartet counts (value order per segment)
+        const std::vector<double> kp = ket_prim_prefix(B);
         double w = 0.0;
         long qn = 0;
         for (int r = 0; r < nsig[A]; r++) {
-            w += (double)h_pnprim[h_sig[A][r]] * kp[exact[r]];
+            const int s0 = seg_start[B][h_sigmol[A][r]];
+            w += (double)h_pnprim[h_sig[A][r]] * (kp[s0 + exact[r]] - kp[s0]);
             qn += exact[r];
         }
         nq[cls] = qn;
         nprimq[cls] = w;
         h_exact[cls] = exact;
-        h_slots[cls].resize(nsig[A]);
-        for (int r = 0; r < nsig[A]; r++) h_slots[cls][r] = off[r + 1] - off[r];
     }
+    dfree(d_counts);
+    dfree(d_exact);
     if (build_tiles()) return -1;
     screened = true;
     return 0;
 }
 
+// prefix sums of primitive-pair counts over a type list in value order
+// (segment by segment): kp[s0 + k] - kp[s0] = sum over the k largest of a segment
+std::vector<double> GtfSystem::ket_prim_prefix(int B) const
+{
+    std::vector<int> byrank(nsig[B]);
+    for (int j = 0; j < nsig[B]; j++) byrank[seg_start[B][h_sigmol[B][j]] + h_rank[B][j]] = h_sig[B][j];
+    std::vector<double> kp(nsig[B] + 1, 0.0);
+    for (int j = 0; j < nsig[B]; j++) kp[j + 1] = kp[j] + h_pnprim[byrank[j]];
+    return kp;
+}
+
+long GtfSystem::quartets(int* out, long cap) const
+{
+    long k = 0;
+    for (int cls = 0; cls < GTF_NCLASSES; cls++) {
+        if (nq[cls] == 0) continue;
+        int A, B;
+        gtf_class_types(cls, A, B);
+        for (int r = 0; r < nsig[A]; r++) {
+            const int m = h_sigmol[A][r];
+            const int s0 = seg_start[B][m];
+            for (int j = s0; j < s0 + seg_len[B][m]; j++) {
+                if (!(h_sigv[A][r] * h_sigv[B][j] >= tol2) || (A == B && h_rank[B][j] > h_rank[A][r])) continue;
+                if (out && k < cap) {
+                    const int bra = h_sig[A][r], ket = h_sig[B][j];
+                    out[4 * k + 0] = h_s1[bra];
+                    out[4 * k + 1] = h_s2[bra];
+                    out[4 * k + 2] = h_s1[ket];
+                    out[4 * k + 3] = h_s2[ket];
+                }
+                k++;
+            }
+        }
+    }
+    return k;
+}
+
 void GtfSystem::csr(std::vector<int>& shellptr, std::vector<int>& shellid, std::vector<int>& shellrid,
                     std::vector<double>& shellvalue) const
 {
-    const double eta = tol2 / vmax;
+    const double eta = tol2 / mol_vmax[0];
+    const int nshells = mol_nshells[0];
     shellptr.assign(nshells + 1, 0);
     shellid.clear();
     shellrid.clear();
@@ -284,10 +410,12 @@ void GtfSystem::csr(std::vector<int>& shellptr, std::vector<int>& shellid, std::
     shellptr[nshells] = (int)shellid.size();
 }
 
-// Tiles of up to GTF_TILE_UNITS work units of one bra row.  With tau > 0 each
-// row splits at the first ket whose Schwarz bound sqrt(v_bra v_ket) < tau:
-// FP64 tiles cover the block-padded prefix, FP32 tiles start at the block
-// holding the split; the kernels re-test the exact predicates per quartet.
+// Tiles of up to GTF_TILE_UNITS work units of one bra row; (bra row, first
+// ket, end ket) with ket positions absolute in the ket type list.  With
+// tau > 0 each row splits at the first ket whose Schwarz bound
+// sqrt(v_bra v_ket) < tau: FP64 tiles cover the block-padded prefix, FP32
+// tiles start at the block holding the split; the kernels re-test the exact
+// predicates per quartet.
 int GtfSystem::build_tiles()
 {
     const double tau2 = tau * tau;
@@ -300,37 +428,36 @@ int GtfSystem::build_tiles()
         ntiles[cls] = ntiles_lo[cls] = 0;
         nq_lo[cls] = 0;
         nprimq_lo[cls] = 0.0;
+        dfree(d_tiles[cls]);
+        dfree(d_tiles_lo[cls]);
+        d_tiles[cls] = d_tiles_lo[cls] = nullptr;
         if (nq[cls] == 0 || !use_tiles || tile_lds[cls] > max_lds) continue;
-        std::vector<double> vs(nsig[B]);
-        std::vector<int> byrank(nsig[B]);
-        for (int j = 0; j < nsig[B]; j++) {
-            vs[h_rank[B][j]] = h_sigv[B][j];
-            byrank[h_rank[B][j]] = h_sig[B][j];
-        }
-        std::vector<double> kp(nsig[B] + 1, 0.0);
-        for (int j = 0; j < nsig[B]; j++) kp[j + 1] = kp[j] + h_pnprim[byrank[j]];
+        const std::vector<double>& vs = h_sorted[B];
+        const std::vector<double> kp = ket_prim_prefix(B);
         const int tq = std::max(1, GTF_TILE_UNITS / ci->nsplit);
         std::vector<int4> hi, lo;
         for (int r = 0; r < nsig[A]; r++) {
             const long c = h_exact[cls][r];
             if (c == 0) continue;
+            const int m = h_sigmol[A][r];
+            const int s0 = seg_start[B][m], nket = seg_len[B][m];
             long c64 = c;
             if (tau2 > 0.0) {
                 const double vk = h_sigv[A][r];
-                c64 = std::upper_bound(vs.begin(), vs.begin() + c, vk, [&](double v, double x) {
+                c64 = std::upper_bound(vs.begin() + s0, vs.begin() + s0 + c, vk, [&](double v, double x) {
                           return !(v * x >= tau2);
-                      }) - vs.begin();
+                      }) - (vs.begin() + s0);
             }
-            const long pad = std::min<long>(nsig[B], ((c + GTF_KET_BLOCK - 1) / GTF_KET_BLOCK) * GTF_KET_BLOCK);
-            const long pad64 = std::min<long>(nsig[B], ((c64 + GTF_KET_BLOCK - 1) / GTF_KET_BLOCK) * GTF_KET_BLOCK);
+            const long pad = std::min<long>(nket, ((c + GTF_KET_BLOCK - 1) / GTF_KET_BLOCK) * GTF_KET_BLOCK);
+            const long pad64 = std::min<long>(nket, ((c64 + GTF_KET_BLOCK - 1) / GTF_KET_BLOCK) * GTF_KET_BLOCK);
             for (long j0 = 0; j0 < pad64; j0 += tq)
-                hi.push_back(make_int4(r, (int)j0, (int)std::min<long>(pad64, j0 + tq), 0));
+                hi.push_back(make_int4(r, (int)(s0 + j0), (int)(s0 + std::min<long>(pad64, j0 + tq)), 0));
             if (c > c64) {
-                const long s0 = (c64 / GTF_KET_BLOCK) * GTF_KET_BLOCK;
-                for (long j0 = s0; j0 < pad; j0 += tq)
-                    lo.push_back(make_int4(r, (int)j0, (int)std::min<long>(pad, j0 + tq), 0));
+                const long b0 = (c64 / GTF_KET_BLOCK) * GTF_KET_BLOCK;
+                for (long j0 = b0; j0 < pad; j0 += tq)
+                    lo.push_back(make_int4(r, (int)(s0 + j0), (int)(s0 + std::min<long>(pad, j0 + tq)), 0));
                 nq_lo[cls] += c - c64;
-                nprimq_lo[cls] += (double)h_pnprim[h_sig[A][r]] * (kp[c] - kp[c64]);
+                nprimq_lo[cls] += (double)h_pnprim[h_sig[A][r]] * (kp[s0 + c] - kp[s0 + c64]);
             }
         }
         ntiles[cls] = (int)hi.size();
@@ -348,7 +475,7 @@ int GtfSystem::build_tiles()
     return 0;
 }
 
-int GtfSystem::jk(const double* D, double* J, double* K, int ld)
+int GtfSystem::jk(const double* D, double* J, double* K)
 {
     if (gtf_fill_int(stream, d_counters, 2 * GTF_NCLASSES, 0)) return -1;
     const PairView pv = pair_view();
@@ -372,7 +499,7 @@ int GtfSystem::jk(const double* D, double* J, double* K, int ld)
         int A, B;
         gtf_class_types(cls, A, B);
         const EriClassInfo* ci = gtf_eri_class(cls);
-        hipStream_t cs = cstream[slot++ % NSTREAMS];
+        hipStream_t cs = serial ? stream : cstream[slot++ % NSTREAMS];
         EriArgs a{};
         a.nunits = nslots[cls] * ci->nsplit;
         a.row_off = d_rowoff[cls];
@@ -393,7 +520,10 @@ int GtfSystem::jk(const double* D, double* J, double* K, int ld)
         a.D = D;
         a.J = J;
         a.K = K;
-        a.ld = ld;
+        a.mbase = d_mbase;
+        a.mld = d_mld;
+        a.bra_mol = d_sigmol[A];
+        a.ket_seg_start = d_seg_start[B];
         if (timing) SYS_CHECK(hipEventRecord(tev[2 * cls], cs));
         if (ntiles[cls] > 0 || ntiles_lo[cls] > 0) {
             a.nbf = nbf;
@@ -420,6 +550,11 @@ int GtfSystem::jk(const double* D, double* J, double* K, int ld)
         SYS_CHECK(hipStreamWaitEvent(stream, join_ev[s], 0));
     }
     return 0;
+}
+
+int GtfSystem::symmetrize(double* A)
+{
+    return gtf_symmetrize_packed(stream, A, nmol, d_mol_off, d_mol_n, nbf);
 }
 
 int GtfSystem::collect_times()

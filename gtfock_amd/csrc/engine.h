@@ -1,7 +1,15 @@
-// engine.h -- device-resident molecular data shared by the PFock engine and
-// the CInt ERD/OED handles: basis arrays, the Boys grid, every shell pair's
-// primitive-pair data, the Schwarz values and the screened quartet rows.
+// engine.h -- device-resident molecular data shared by the PFock engine, the
+// batched engine and the CInt ERD/OED handles: basis arrays, the Boys grid,
+// every shell pair's primitive-pair data, the Schwarz values and the
+// screened quartet tiles.
+//
+// One GtfSystem holds one or more molecules.  Shells, atoms and pairs are
+// numbered globally; pairs and quartets never mix molecules.  Each
+// molecule's D / J / K live in packed n_m x n_m blocks (matrix base and
+// leading dimension per shell), so a batch of molecules is one set of
+// arrays and one launch per quartet class.
 #pragma once
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -11,87 +19,116 @@
 
 struct GtfSystem {
     hipStream_t stream = nullptr;
-    int nbf = 0, nshells = 0, natoms = 0;
-    // device basis
+    // ---- molecules
+    int nmol = 0;
+    int nbf = 0;      // largest molecule (strip width, dense API)
+    int nshells = 0;  // total
+    int natoms = 0;   // total
+    std::vector<int> mol_nbf, mol_shell_off, mol_nshells, mol_pair_off, mol_npairs;
+    std::vector<long> mol_mat_off;  // offset of the molecule's n x n block
+    long mat_total = 0;             // sum of n_m^2
+    std::vector<int> h_shell_mol, h_pair_mol;
+    // ---- device basis (global shell / atom numbering; fstart is local)
     int *d_L = nullptr, *d_nprim = nullptr, *d_poff = nullptr, *d_fstart = nullptr, *d_atom = nullptr;
     double *d_alpha = nullptr, *d_coef = nullptr, *d_center = nullptr, *d_atom_xyz = nullptr, *d_atom_Z = nullptr;
+    long* d_mbase = nullptr;  // per shell: matrix block offset of its molecule
+    int* d_mld = nullptr;     // per shell: leading dimension (n of its molecule)
+    long* d_mol_off = nullptr;
+    int* d_mol_n = nullptr;
     double* d_boys = nullptr;
-    // all unordered shell pairs, canonical orientation L(s1) >= L(s2)
+    // ---- shell pairs of every molecule, canonical orientation L(s1) >= L(s2)
     int npairs = 0;
     std::vector<int> h_s1, h_s2, h_type, h_pnprim;
     int *d_s1 = nullptr, *d_s2 = nullptr, *d_pnprim = nullptr, *d_ppoff = nullptr;
     long* d_peoff = nullptr;
     double4* d_prim = nullptr;
     double* d_E = nullptr;
+    long* d_pzoff = nullptr;
+    double* d_Ez = nullptr;
     long nprimpairs = 0, nE = 0, nprimpairs_kept = 0;
     // primitive-pair screening threshold on max |E| of a pair block
     double prim_eps = 1e-20;
-    // Schwarz values per pair and screening results
+    // ---- Schwarz values and screening
     std::vector<double> h_v;
-    double vmax = 0.0;
+    std::vector<double> mol_vmax;
+    double vmax = 0.0;  // of molecule 0
     double tol = 0.0, tol2 = 0.0;
     bool screened = false;
-    // significant pairs per type, sorted by Schwarz value (descending)
+    // significant pairs per type: one segment per molecule, each segment
+    // value-sorted (descending) and then re-sorted by primitive count within
+    // blocks of GTF_KET_BLOCK; rank = position in the segment's value order
     int nsig[GTF_NPAIRTYPES] = {0};
-    std::vector<int> h_sig[GTF_NPAIRTYPES];
+    std::vector<int> h_sig[GTF_NPAIRTYPES], h_rank[GTF_NPAIRTYPES], h_sigmol[GTF_NPAIRTYPES];
+    std::vector<double> h_sigv[GTF_NPAIRTYPES], h_sorted[GTF_NPAIRTYPES];
+    std::vector<int> seg_start[GTF_NPAIRTYPES], seg_len[GTF_NPAIRTYPES];
     int* d_sig[GTF_NPAIRTYPES] = {nullptr};
     double* d_sigv[GTF_NPAIRTYPES] = {nullptr};
-    // the lists are value-sorted, then each block of GTF_KET_BLOCK entries is
-    // re-sorted by primitive count; rank = position in the value order and
-    // sigv_sorted = the values in value order (for the row counts)
-    std::vector<double> h_sigv[GTF_NPAIRTYPES];
-    std::vector<int> h_rank[GTF_NPAIRTYPES];
     int* d_rank[GTF_NPAIRTYPES] = {nullptr};
+    int* d_sigmol[GTF_NPAIRTYPES] = {nullptr};
     double* d_sigv_sorted[GTF_NPAIRTYPES] = {nullptr};
-    // quartet rows per class
+    int* d_seg_start[GTF_NPAIRTYPES] = {nullptr};
+    int* d_seg_len[GTF_NPAIRTYPES] = {nullptr};
+    // ---- quartet rows and tiles per class
     long nq[GTF_NCLASSES] = {0};      // screened unique quartets
     long nslots[GTF_NCLASSES] = {0};  // row slots incl. the block padding
     long* d_rowoff[GTF_NCLASSES] = {nullptr};
-    int* d_counters = nullptr;
-    // class kernels run concurrently on these streams (forked from and joined
-    // to the caller's stream with events)
-    static constexpr int NSTREAMS = 4;
-    hipStream_t cstream[NSTREAMS] = {nullptr};
-    hipEvent_t fork_ev = nullptr, join_ev[NSTREAMS] = {nullptr};
-    size_t dev_bytes = 0;
-    // bra-row tiles per class for the LDS-strip kernels (empty: global-atomic
-    // kernel); tiles[cls] are the FP64 tiles, tiles_lo[cls] the FP32 ones
-    int ntiles[GTF_NCLASSES] = {0};
-    int4* d_tiles[GTF_NCLASSES] = {nullptr};
-    int ntiles_lo[GTF_NCLASSES] = {0};
-    int4* d_tiles_lo[GTF_NCLASSES] = {nullptr};
-    std::vector<long> h_exact[GTF_NCLASSES], h_slots[GTF_NCLASSES];
+    std::vector<long> h_exact[GTF_NCLASSES];
+    double nprimq[GTF_NCLASSES] = {0};  // sum over quartets of nprim(bra) nprim(ket)
+    int ntiles[GTF_NCLASSES] = {0}, ntiles_lo[GTF_NCLASSES] = {0};
+    int4* d_tiles[GTF_NCLASSES] = {nullptr};     // FP64 tiles
+    int4* d_tiles_lo[GTF_NCLASSES] = {nullptr};  // FP32 tiles
+    size_t tile_lds[GTF_NCLASSES] = {0};
+    size_t max_lds = 65536;
+    bool use_tiles = true;
     // mixed precision: Schwarz bound threshold tau (0 = plain FP64)
     double tau = 0.0;
     long nq_lo[GTF_NCLASSES] = {0};
     double nprimq_lo[GTF_NCLASSES] = {0};
-    int build_tiles();
-    size_t tile_lds[GTF_NCLASSES] = {0};
-    size_t max_lds = 65536;
-    bool use_tiles = true;
-    // primitive quartets per class (sum over quartets of nprim(bra) * nprim(ket))
-    double nprimq[GTF_NCLASSES] = {0};
-    // optional per-class kernel timing with events on the launch stream
-    bool timing = false;
+    int* d_counters = nullptr;
+    // ---- launch machinery: class kernels run concurrently on NSTREAMS
+    // streams forked from and joined to the caller's stream with events
+    static constexpr int NSTREAMS = 4;
+    hipStream_t cstream[NSTREAMS] = {nullptr};
+    hipEvent_t fork_ev = nullptr, join_ev[NSTREAMS] = {nullptr};
+    bool serial = false;  // all class kernels on the caller's stream (profiling)
+    bool timing = false;  // per-class events
     hipEvent_t tev[2 * GTF_NCLASSES] = {nullptr};
     double class_ms[GTF_NCLASSES] = {0};
     long class_launches[GTF_NCLASSES] = {0};
+    size_t dev_bytes = 0;
+    std::vector<void*> allocs;
 
-    int init(const BasisSet* b, hipStream_t st);
+    int init(const std::vector<const BasisSet*>& mols, hipStream_t st);
+    int init(const BasisSet* b, hipStream_t st) { return init(std::vector<const BasisSet*>{b}, st); }
     // Schwarz screening (pfock/screening.c:15-180) and quartet compaction
     int screen(double tolscr);
-    // CSR in the reference layout (host)
+    int build_tiles();
+    // CSR of molecule 0 in the reference layout (host)
     void csr(std::vector<int>& shellptr, std::vector<int>& shellid, std::vector<int>& shellrid,
              std::vector<double>& shellvalue) const;
-    int pair_id(int A, int B) const { return A >= B ? A * (A + 1) / 2 + B : B * (B + 1) / 2 + A; }
+    // global pair id of two shells of the same molecule
+    int pair_id(int A, int B) const
+    {
+        const int m = h_shell_mol[A];
+        int a = A - mol_shell_off[m], b = B - mol_shell_off[m];
+        if (a < b) std::swap(a, b);
+        return mol_pair_off[m] + a * (a + 1) / 2 + b;
+    }
     PairView pair_view() const;
-    // fused ERI + J/K digestion over all screened quartets of every class
-    int jk(const double* D, double* J, double* K, int ld);
-    // accumulate the per-class event times of the last jk() into class_ms
+    // fused ERI + J/K digestion over all screened quartets of every class;
+    // D, J, K are the packed blocks of all molecules
+    int jk(const double* D, double* J, double* K);
     int collect_times();
+    // A <- (A + A^T)/2 per molecule block
+    int symmetrize(double* A);
     // contracted integrals of a list of (bra, ket) canonical pair quartets of one class
     int store(int cls, int nq_, const int2* d_qlist, double* d_out);
+    // screened unique quartets as canonical (M, N, P, Q) global shell ids;
+    // returns the count, writes at most cap
+    long quartets(int* out, long cap) const;
+    std::vector<double> ket_prim_prefix(int B) const;
     void release();
+    void dfree(void* p);
     ~GtfSystem() { release(); }
 
     template <typename T>
@@ -104,11 +141,8 @@ struct GtfSystem {
         dev_bytes += n * sizeof(T);
         return (T*)p;
     }
-    std::vector<void*> allocs;
 };
 
-// pair type of a canonical shell pair
-inline int gtf_pair_type(int la, int lb) { return pair_type(la, lb); }
 void gtf_class_types(int cls, int& A, int& B);
 
 // status reporting in the style of PFOCK_PRINTF (pfock/config.h:17-29)

@@ -36,7 +36,6 @@ struct EriArgs {
     const double* D;
     double* J;
     double* K;
-    int ld;
     // EPI_DIAG
     unsigned long long* vbits;
     // EPI_STORE
@@ -44,7 +43,7 @@ struct EriArgs {
     // bra-row tiles of the LDS-strip J/K kernel: (bra row, first ket, end ket)
     const int4* tiles;
     int ntiles;
-    int nbf;
+    int nbf;  // LDS strip capacity (largest molecule)
     // exact screening of the block-padded rows
     const double* vbra;
     const double* vket;
@@ -54,6 +53,12 @@ struct EriArgs {
     int same;
     // mixed precision: FP64 quartets have v_bra * v_ket >= tau2, FP32 the rest
     double tau2;
+    // packed per-molecule matrices: block offset and leading dimension per shell
+    const long* mbase;
+    const int* mld;
+    // global-atomic path: molecule of each bra entry, ket segment start per molecule
+    const int* bra_mol;
+    const int* ket_seg_start;
 };
 
 // the Schwarz test of pfock/fock_task.c:192 and, for a diagonal class, the
@@ -99,7 +104,7 @@ __device__ __forceinline__ void eri_unit(const EriArgs& a, long u, const double*
         ket = bk.y;
     } else {
         const int r = find_row(a.row_off, a.nrows, q);
-        const int j = (int)(q - a.row_off[r]);
+        const int j = a.ket_seg_start[a.bra_mol[r]] + (int)(q - a.row_off[r]);
         if (!quartet_valid(a, r, j)) return;
         bra = a.bra_list[r];
         ket = a.ket_list[j];
@@ -150,11 +155,12 @@ __device__ __forceinline__ void digest(const EriArgs& a, int bra, int ket, int c
     const double cJ1 = 2.0 * (1 + f1) * (1 + f2);
     const double cJ2 = 2.0 * f3 * (1 + f1) * (1 + f2);
     const double cMP = (1 + f3), cNP = f1 * (1 + f3), cMQ = f2 * (1 + f3), cNQ = f1 * f2 * (1 + f3);
-    const int ld = a.ld;
+    const int ld = a.mld[M];
+    const long base = a.mbase[M];
     const int oM = a.fstart[M], oN = a.fstart[N], oP = a.fstart[P], oQ = a.fstart[Q];
-    const double* __restrict__ D = a.D;
-    double* J = a.J;
-    double* K = a.K;
+    const double* __restrict__ D = a.D + base;
+    double* J = a.J + base;
+    double* K = a.K + base;
     int mloc[NC], nloc[NC];
     double dmn[NC];
     static_for<0, NC>([&](auto c_) {
@@ -270,7 +276,7 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
 // GTF_TILE_UNITS work units of one bra row; the workgroup accumulates J_MN and
 // the exchange blocks of the bra shells in LDS and flushes them with one HBM
 // atomic per touched element when the tile is done.  Dynamic LDS:
-// (NA + NB) * nbf + NA * NB doubles.
+// (NA + NB) * nbf + NA * NB doubles, nbf = the largest molecule's size.
 #define GTF_TILE_UNITS 1024
 
 template <int LA, int LB, int LC, int LD, int NC, typename Real>
@@ -282,10 +288,6 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
     constexpr bool LOW = std::is_same<Real, float>::value;
     extern __shared__ double smem[];
     __shared__ int s_task;
-    const int nbf = a.nbf;
-    const int nstrip = (NA + NB) * nbf;
-    double* strip = smem;
-    double* jmn = smem + nstrip;
     for (;;) {
         if (threadIdx.x == 0) s_task = atomicAdd(a.counter, 1);
         __syncthreads();
@@ -293,6 +295,11 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
         if (task >= a.ntiles) break;
         const int4 t = a.tiles[task];
         const int bra = a.bra_list[t.x];
+        // strip rows are as wide as the bra's molecule (at most a.nbf)
+        const int nbf = a.mld[a.pv.s1[bra]];
+        const int nstrip = (NA + NB) * nbf;
+        double* strip = smem;
+        double* jmn = smem + nstrip;
         for (int k = threadIdx.x; k < nstrip + NA * NB; k += 256) smem[k] = 0.0;
         __syncthreads();
         const int nunits = (t.z - t.y) * NSPLIT;
@@ -321,17 +328,19 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
         __syncthreads();
         const int M = a.pv.s1[bra], N = a.pv.s2[bra];
         const int oM = a.fstart[M], oN = a.fstart[N];
+        const long base = a.mbase[M];
+        const int ld = a.mld[M];
         for (int k = threadIdx.x; k < nstrip; k += 256) {
             const double v = strip[k];
             if (v != 0.0) {
                 const int r = k / nbf, col = k - r * nbf;
                 const int row = r < NA ? oM + r : oN + (r - NA);
-                GTF_ATOMIC_ADD(&a.K[(long)row * a.ld + col], v);
+                GTF_ATOMIC_ADD(&a.K[base + (long)row * ld + col], v);
             }
         }
         for (int k = threadIdx.x; k < NA * NB; k += 256) {
             const double v = jmn[k];
-            if (v != 0.0) GTF_ATOMIC_ADD(&a.J[(long)(oM + k / NB) * a.ld + oN + k % NB], v);
+            if (v != 0.0) GTF_ATOMIC_ADD(&a.J[base + (long)(oM + k / NB) * ld + oN + k % NB], v);
         }
         __syncthreads();
     }

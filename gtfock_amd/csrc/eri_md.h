@@ -87,6 +87,9 @@ __host__ __device__ constexpr int default_nc()
 
 // Contracted integrals (bra components c0..c0+NC-1, all ket components) of
 // the quartet (bra pair | ket pair).  acc[c][cd], cd = c_sph * nsph(LD) + d_sph.
+// The ket coefficients are read in the compact nonzero-only layout and the
+// structurally zero terms are skipped at compile time; when a thread owns the
+// whole bra pair (NC == NAB, c0 == 0) the bra side is treated the same way.
 template <int LA, int LB, int LC, int LD, int NC, typename Real, int BSTRIDE = BOYS_NCOL>
 __device__ __forceinline__ void eri_block(const PairView& pv, int bra, int ket, int c0,
                                           const double* __restrict__ boys_tab,
@@ -94,14 +97,18 @@ __device__ __forceinline__ void eri_block(const PairView& pv, int bra, int ket, 
 {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr int L = S::L, HAB = S::HAB, HCD = S::HCD, HL = S::HL, NCD = S::NCD;
+    constexpr bool BRA_FULL = (NC == S::NAB);
+    using TA = NzTab<LA, LB>;
+    using TC = NzTab<LC, LD>;
+    constexpr int NZAB = TA::data.nnz, NZCD = TC::data.nnz;
     static_for<0, NC>([&](auto c_) {
         static_for<0, NCD>([&](auto d_) { acc[decltype(c_)::value][decltype(d_)::value] = Real(0); });
     });
     const int nb = pv.nprim[bra], nk = pv.nprim[ket];
     const double4* __restrict__ pb = pv.prim + pv.poff[bra];
     const double4* __restrict__ pk = pv.prim + pv.poff[ket];
-    const double* __restrict__ Eb = pv.E + pv.eoff[bra] + (long)c0 * HAB;
-    const double* __restrict__ Ek0 = pv.E + pv.eoff[ket];
+    const double* __restrict__ Eb = BRA_FULL ? pv.Ez + pv.zoff[bra] : pv.E + pv.eoff[bra] + (long)c0 * HAB;
+    const double* __restrict__ Ek0 = pv.Ez + pv.zoff[ket];
     for (int i = 0; i < nb; ++i) {
         const double4 B = pb[i];
         Real eb[NC][HAB];
@@ -109,7 +116,11 @@ __device__ __forceinline__ void eri_block(const PairView& pv, int bra, int ket, 
             constexpr int c = decltype(c_)::value;
             static_for<0, HAB>([&](auto h_) {
                 constexpr int h = decltype(h_)::value;
-                eb[c][h] = Real(Eb[(long)i * S::ESZ_AB + c * HAB + h]);
+                if constexpr (BRA_FULL) {
+                    if constexpr (TA::data.nz[c][h]) eb[c][h] = Real(Eb[(long)i * NZAB + TA::data.pos[c][h]]);
+                } else {
+                    eb[c][h] = Real(Eb[(long)i * S::ESZ_AB + c * HAB + h]);
+                }
             });
         });
         for (int j = 0; j < nk; ++j) {
@@ -133,21 +144,24 @@ __device__ __forceinline__ void eri_block(const PairView& pv, int bra, int ket, 
                     Real v = Real(0);
                     static_for<0, HAB>([&](auto h1_) {
                         constexpr int h1 = decltype(h1_)::value;
-                        v += eb[c][h1] * R[hsum<h1, h2>()];
+                        if constexpr (!BRA_FULL || TA::data.nz[c][h1])
+                            v += eb[c][h1] * R[hsum<h1, h2>()];
                     });
                     V[c][h2] = neg ? -v : v;
                 });
             });
-            const double* __restrict__ ek = Ek0 + (long)j * S::ESZ_CD;
+            const double* __restrict__ ek = Ek0 + (long)j * NZCD;
             static_for<0, NCD>([&](auto cd_) {
                 constexpr int cd = decltype(cd_)::value;
                 static_for<0, HCD>([&](auto h2_) {
                     constexpr int h2 = decltype(h2_)::value;
-                    const Real e = Real(ek[cd * HCD + h2]);
-                    static_for<0, NC>([&](auto c_) {
-                        constexpr int c = decltype(c_)::value;
-                        acc[c][cd] += e * V[c][h2];
-                    });
+                    if constexpr (TC::data.nz[cd][h2]) {
+                        const Real e = Real(ek[TC::data.pos[cd][h2]]);
+                        static_for<0, NC>([&](auto c_) {
+                            constexpr int c = decltype(c_)::value;
+                            acc[c][cd] += e * V[c][h2];
+                        });
+                    }
                 });
             });
         }

@@ -114,6 +114,8 @@ struct PairView {
     const long* eoff;    // offset into E[]
     const double4* prim; // (p, Px, Py, Pz) per primitive pair
     const double* E;     // Hermite->spherical coefficients, scaled by c_i c_j K_ab / p
+    const long* zoff;    // offset into Ez[]
+    const double* Ez;    // the same, structural nonzeros only (herm_nz_pos order)
 };
 
 struct BasisView {
@@ -132,7 +134,7 @@ struct BasisView {
 
 // Spherical transform rows (m = -l..l) over cartesian components of order l
 // in the hidx order (xx, xy, xz, yy, yz, zz for l = 2).
-__host__ __device__ inline double sph_coef(int l, int m, int c)
+__host__ __device__ constexpr double sph_coef(int l, int m, int c)
 {
     if (l == 0) return 1.0;
     if (l == 1) return m == c ? 1.0 : 0.0;
@@ -147,5 +149,68 @@ __host__ __device__ inline double sph_coef(int l, int m, int c)
     return 0.0;
 }
 
-// host-side entry points of the kernel files
-struct EriLaunch;
+// Structural nonzeros of the Hermite coefficients of a spherical component
+// pair (as, bs) of shells (la, lb): E^{ab}_{tuv} can be nonzero only if some
+// cartesian pair (a, b) in the two spherical functions has t <= ax + bx,
+// u <= ay + by, v <= az + bz.
+__host__ __device__ constexpr bool herm_nz(int la, int lb, int as, int bs, int h)
+{
+    const int t = herm_t(h), u = herm_u(h), v = herm_v(h);
+    const int ca0 = la == 0 ? 0 : nherm(la - 1), cb0 = lb == 0 ? 0 : nherm(lb - 1);
+    for (int ca = 0; ca < ncart(la); ca++) {
+        if (sph_coef(la, as, ca) == 0.0) continue;
+        for (int cb = 0; cb < ncart(lb); cb++) {
+            if (sph_coef(lb, bs, cb) == 0.0) continue;
+            if (t <= herm_t(ca0 + ca) + herm_t(cb0 + cb) && u <= herm_u(ca0 + ca) + herm_u(cb0 + cb) &&
+                v <= herm_v(ca0 + ca) + herm_v(cb0 + cb))
+                return true;
+        }
+    }
+    return false;
+}
+// The structural-nonzero table of one pair type, evaluated once at compile
+// time: nz[ab][h], pos[ab][h] (position in the compact layout), nnz.
+template <int LA, int LB>
+struct NzTab {
+    static constexpr int NAB = nsph(LA) * nsph(LB), H = nherm(LA + LB);
+    struct Data {
+        bool nz[NAB][H];
+        int pos[NAB][H];
+        int nnz;
+    };
+    static constexpr Data make()
+    {
+        Data d{};
+        int n = 0;
+        for (int ab = 0; ab < NAB; ab++)
+            for (int h = 0; h < H; h++) {
+                d.nz[ab][h] = herm_nz(LA, LB, ab / nsph(LB), ab % nsph(LB), h);
+                d.pos[ab][h] = n;
+                if (d.nz[ab][h]) n++;
+            }
+        d.nnz = n;
+        return d;
+    }
+    static constexpr Data data = make();
+};
+
+// number of structural nonzeros of a (la, lb) pair block (runtime form), and
+// the position of (component ab, hermite h) in the compact layout
+__host__ __device__ constexpr int herm_nnz(int la, int lb)
+{
+    int n = 0;
+    for (int ab = 0; ab < nsph(la) * nsph(lb); ab++)
+        for (int h = 0; h < nherm(la + lb); h++)
+            if (herm_nz(la, lb, ab / nsph(lb), ab % nsph(lb), h)) n++;
+    return n;
+}
+__host__ __device__ constexpr int herm_nz_pos(int la, int lb, int ab, int h)
+{
+    int n = 0;
+    for (int x = 0; x < nsph(la) * nsph(lb); x++)
+        for (int y = 0; y < nherm(la + lb); y++) {
+            if (x == ab && y == h) return n;
+            if (herm_nz(la, lb, x / nsph(lb), x % nsph(lb), y)) n++;
+        }
+    return n;
+}

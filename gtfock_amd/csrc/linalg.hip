@@ -112,6 +112,30 @@ __global__ void symmetrize_kernel(double* A, int n, int ld)
 
 static int grid_for(long n) { return (int)std::min<long>((n + 255) / 256, 4096); }
 
+// per-molecule blocks of a packed array: A_m <- (A_m + A_m^T)/2
+__global__ void symmetrize_packed_kernel(double* A, const long* __restrict__ off, const int* __restrict__ nn)
+{
+    const int m = blockIdx.y;
+    const int n = nn[m];
+    double* B = A + off[m];
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < (long)n * n; k += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(k / n), j = (int)(k - (long)i * n);
+        if (j >= i) continue;
+        const double s = 0.5 * (B[(long)i * n + j] + B[(long)j * n + i]);
+        B[(long)i * n + j] = s;
+        B[(long)j * n + i] = s;
+    }
+}
+
+int gtf_symmetrize_packed(hipStream_t st, double* A, int nmol, const long* off, const int* n, int maxn)
+{
+    if (nmol <= 0) return 0;
+    const int gx = (int)std::min<long>(((long)maxn * maxn + 255) / 256, 1024);
+    symmetrize_packed_kernel<<<dim3(gx, nmol), 256, 0, st>>>(A, off, n);
+    GTF_CHECK(hipGetLastError());
+    return 0;
+}
+
 int gtf_fill(hipStream_t st, double* x, long n, double v)
 {
     if (n <= 0) return 0;

@@ -336,10 +336,9 @@ PFockStatus_t PFock_computeFock(BasisSet_t basis, PFock_t pf)
     for (int i = 0; i < pf->num_dmat; i++) {
         if (gtf_fill(pf->stream, pf->d_J[i], nn, 0.0) || gtf_fill(pf->stream, pf->d_K[i], nn, 0.0))
             return PFOCK_STATUS_EXECUTION_FAILED;
-        if (pf->sys.jk(pf->d_D[i], pf->d_J[i], pf->d_K[i], pf->nbf)) return PFOCK_STATUS_EXECUTION_FAILED;
+        if (pf->sys.jk(pf->d_D[i], pf->d_J[i], pf->d_K[i])) return PFOCK_STATUS_EXECUTION_FAILED;
         if (pf->sys.collect_times()) return PFOCK_STATUS_EXECUTION_FAILED;
-        if (gtf_symmetrize(pf->stream, pf->d_J[i], pf->nbf, pf->nbf) ||
-            gtf_symmetrize(pf->stream, pf->d_K[i], pf->nbf, pf->nbf))
+        if (pf->sys.symmetrize(pf->d_J[i]) || pf->sys.symmetrize(pf->d_K[i]))
             return PFOCK_STATUS_EXECUTION_FAILED;
     }
     PF_HIP(hipEventRecord(pf->ev1, pf->stream), "hipEventRecord");
@@ -534,33 +533,7 @@ int gtf_pfock_class_counts(PFock_t pf, long* counts)
 }
 
 // unique screened quartets as canonical (M,N,P,Q) shell ids, class by class
-long gtf_pfock_quartets(PFock_t pf, int* out, long cap)
-{
-    GtfSystem& s = pf->sys;
-    long k = 0;
-    for (int cls = 0; cls < GTF_NCLASSES; cls++) {
-        if (s.nq[cls] == 0) continue;
-        int A, B;
-        gtf_class_types(cls, A, B);
-        std::vector<long> off(s.nsig[A] + 1);
-        if (hipMemcpy(off.data(), s.d_rowoff[cls], sizeof(long) * off.size(), hipMemcpyDeviceToHost) != hipSuccess)
-            return -1;
-        for (int r = 0; r < s.nsig[A]; r++)
-            for (long j = 0; j < off[r + 1] - off[r]; j++) {
-                if (!(s.h_sigv[A][r] * s.h_sigv[B][j] >= s.tol2) || (A == B && s.h_rank[B][j] > s.h_rank[A][r]))
-                    continue;
-                if (out && k < cap) {
-                    const int bra = s.h_sig[A][r], ket = s.h_sig[B][j];
-                    out[4 * k + 0] = s.h_s1[bra];
-                    out[4 * k + 1] = s.h_s2[bra];
-                    out[4 * k + 2] = s.h_s1[ket];
-                    out[4 * k + 3] = s.h_s2[ket];
-                }
-                k++;
-            }
-    }
-    return k;
-}
+long gtf_pfock_quartets(PFock_t pf, int* out, long cap) { return pf->sys.quartets(out, cap); }
 
 // the engine's device buffers of D / J / K (borrowed; valid until destroy)
 int gtf_pfock_device_ptrs(PFock_t pf, int index, double** D, double** J, double** K)

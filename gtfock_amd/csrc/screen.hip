@@ -18,7 +18,8 @@
 __global__ void build_pairs_kernel(BasisView b, int npairs, const int* __restrict__ s1,
                                    const int* __restrict__ s2, const int* __restrict__ poff,
                                    const long* __restrict__ eoff, double4* __restrict__ prim,
-                                   double* __restrict__ E, int* __restrict__ nprim_out, double prim_eps)
+                                   double* __restrict__ E, const long* __restrict__ zoff, double* __restrict__ Ez,
+                                   int* __restrict__ nprim_out, double prim_eps)
 {
     const int pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= npairs) return;
@@ -59,6 +60,8 @@ __global__ void build_pairs_kernel(BasisView b, int npairs, const int* __restric
             }
             const double scale = b.coef[b.poff[A] + i] * b.coef[b.poff[B] + j] * exp(-mu * AB2) / p;
             double* out = E + eoff[pid] + (long)k * esz;
+            double* outz = Ez + zoff[pid] + (long)k * herm_nnz(la, lb);
+            int nzpos = 0;
             double emax = 0.0;
             const int ca0 = la == 0 ? 0 : nherm(la - 1), cb0 = lb == 0 ? 0 : nherm(lb - 1);
             for (int as = 0; as < na; as++)
@@ -79,6 +82,7 @@ __global__ void build_pairs_kernel(BasisView b, int npairs, const int* __restric
                             }
                         }
                         out[(as * nb + bs) * H + h] = scale * s;
+                        if (herm_nz(la, lb, as, bs, h)) outz[nzpos++] = scale * s;
                         emax = fmax(emax, fabs(scale * s));
                     }
             if (emax >= prim_eps) {
@@ -89,22 +93,26 @@ __global__ void build_pairs_kernel(BasisView b, int npairs, const int* __restric
     nprim_out[pid] = k;
 }
 
-// exact[k] = number of ket pairs j (in value order) with
+// exact[k] = number of ket pairs j of the bra's molecule (value order) with
 // v_bra[k] * v_ket[j] >= tol2 (the test of pfock/fock_task.c:192); for a
 // diagonal class only rank(j) <= rank(k), so each unordered pair of pairs
-// appears once.  counts[k] rounds exact[k] up to whole ket blocks (the kernel
-// re-applies the exact test to the padding).
+// appears once.  counts[k] rounds exact[k] up to whole ket blocks within the
+// molecule's segment (the kernels re-apply the exact test to the padding).
 __global__ void count_rows_kernel(int nbra, const double* __restrict__ vbra, const int* __restrict__ rank_bra,
-                                  int nket, const double* __restrict__ vket_sorted, double tol2, int same,
-                                  long* __restrict__ exact, long* __restrict__ counts)
+                                  const int* __restrict__ bra_mol, const double* __restrict__ vket_sorted,
+                                  const int* __restrict__ seg_start, const int* __restrict__ seg_len, double tol2,
+                                  int same, long* __restrict__ exact, long* __restrict__ counts)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nbra) return;
     const double vk = vbra[k];
+    const int m = bra_mol[k];
+    const double* vs = vket_sorted + seg_start[m];
+    const int nket = seg_len[m];
     int lo = 0, hi = nket;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (vk * vket_sorted[mid] >= tol2) lo = mid + 1;
+        if (vk * vs[mid] >= tol2) lo = mid + 1;
         else hi = mid;
     }
     long cnt = lo;
@@ -150,21 +158,23 @@ __global__ void fill_qlist_diag_kernel(int n, const int* __restrict__ ids, int2*
 // ---- host launchers -------------------------------------------------------
 
 int gtf_build_pairs(hipStream_t st, const BasisView& b, int npairs, const int* s1, const int* s2,
-                    const int* poff, const long* eoff, double4* prim, double* E, int* nprim_out, double prim_eps)
+                    const int* poff, const long* eoff, double4* prim, double* E, const long* zoff, double* Ez,
+                    int* nprim_out, double prim_eps)
 {
     if (npairs <= 0) return 0;
-    build_pairs_kernel<<<(npairs + 127) / 128, 128, 0, st>>>(b, npairs, s1, s2, poff, eoff, prim, E, nprim_out,
-                                                           prim_eps);
+    build_pairs_kernel<<<(npairs + 127) / 128, 128, 0, st>>>(b, npairs, s1, s2, poff, eoff, prim, E, zoff, Ez,
+                                                           nprim_out, prim_eps);
     GTF_CHECK(hipGetLastError());
     return 0;
 }
 
-int gtf_count_rows(hipStream_t st, int nbra, const double* vbra, const int* rank_bra, int nket,
-                   const double* vket_sorted, double tol2, int same, long* exact, long* counts, long* row_off)
+int gtf_count_rows(hipStream_t st, int nbra, const double* vbra, const int* rank_bra, const int* bra_mol,
+                   const double* vket_sorted, const int* seg_start, const int* seg_len, double tol2, int same,
+                   long* exact, long* counts, long* row_off)
 {
     if (nbra <= 0) return 0;
-    count_rows_kernel<<<(nbra + 255) / 256, 256, 0, st>>>(nbra, vbra, rank_bra, nket, vket_sorted, tol2, same,
-                                                          exact, counts);
+    count_rows_kernel<<<(nbra + 255) / 256, 256, 0, st>>>(nbra, vbra, rank_bra, bra_mol, vket_sorted, seg_start,
+                                                          seg_len, tol2, same, exact, counts);
     GTF_CHECK(hipGetLastError());
     scan_kernel<<<1, 1024, 0, st>>>(counts, row_off, nbra);
     GTF_CHECK(hipGetLastError());

@@ -136,3 +136,19 @@ def water_cluster(k, rng, min_oo=2.7):
             v = R @ np.array(xyz) + p
             atoms.append((s, tuple(v)))
     return atoms
+
+
+def batch_molecules(nmol, seed=20261202, max_alkane=20, max_cluster=16):
+    """A seeded batch of nmol molecules: the first half all-trans alkanes
+    CnH2n+2 with n uniform in 1..max_alkane, the second half water clusters
+    of k uniform in 1..max_cluster monomers.  Returns [(name, atoms)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    na = nmol // 2
+    for _ in range(na):
+        n = int(rng.integers(1, max_alkane + 1))
+        out.append(("C%dH%d" % (n, 2 * n + 2), alkane(n)))
+    for _ in range(nmol - na):
+        k = int(rng.integers(1, max_cluster + 1))
+        out.append(("(H2O)%d" % k, water_cluster(k, rng)))
+    return out

@@ -68,6 +68,51 @@ int gtf_erd_quartets(ERD_t erd, int nq, const int *quartets, double *out);
  * transB = 0, C = A * B^T (the pdgemm3D_2 convention) when transB = 1. */
 int gtf_dgemm_host(int n, const double *A, const double *B, double *C, int transB);
 
+/* ---- batches: many molecules per Fock build and per SCF iteration ------
+ *
+ * A batch screens every molecule on its own (tolerance tolscr relative to
+ * that molecule's largest Schwarz value, as pfock/screening.c:111) and
+ * evaluates the screened quartets of all molecules in one launch per
+ * quartet class, replayed as one HIP graph per build.  Matrices of the
+ * batch are packed: molecule m is the row-major nbf[m] x nbf[m] block at
+ * offset[m] (gtf_batch_layout); m < 0 in the put/get calls means the whole
+ * packed array.  Return codes: 0 success, -1 HIP failure, -2 out of
+ * device memory, -3 invalid argument. */
+typedef struct GtfBatch *gtf_batch_t;
+
+int gtf_batch_create(BasisSet_t *basis, int nmol, double tolscr, gtf_batch_t *batch);
+void gtf_batch_destroy(gtf_batch_t batch);
+/* nbf[nmol] and offset[nmol] (either may be NULL); returns nmol */
+int gtf_batch_layout(gtf_batch_t batch, int *nbf, long *offset);
+/* total doubles of a packed array */
+long gtf_batch_total(gtf_batch_t batch);
+/* FP32 ERIs for quartets with Schwarz bound below tau (0: FP64 only) */
+int gtf_batch_set_precision(gtf_batch_t batch, double tau);
+/* 1 (default): replay the build as a captured HIP graph; 0: eager launches */
+int gtf_batch_use_graph(gtf_batch_t batch, int on);
+int gtf_batch_put_density(gtf_batch_t batch, int m, const double *D);
+/* G_m = 2 J(D_m) - K(D_m) for every molecule (the F of PFock_getMat) */
+int gtf_batch_compute_fock(gtf_batch_t batch);
+int gtf_batch_get_fock(gtf_batch_t batch, int m, double *G);
+int gtf_batch_get_density(gtf_batch_t batch, int m, double *D);
+/* borrowed device pointers of the packed D and G arrays */
+int gtf_batch_device_ptrs(gtf_batch_t batch, double **D, double **G);
+void *gtf_batch_stream(gtf_batch_t batch);
+/* out[11] = {quartets, primitive quartets, FP32 quartets, FP32 primitive
+ *            quartets, GPU ms of the last build, graph captured (0/1),
+ *            device bytes, create seconds, SCF seconds in Fock builds,
+ *            in DIIS, in purification}; returns 11 */
+int gtf_batch_stats(gtf_batch_t batch, double *out);
+/* per class (21): quartets, primitive quartets, and their FP32 parts */
+int gtf_batch_class_work(gtf_batch_t batch, long *nquartets, double *nprimquartets, long *nq_fp32,
+                         double *npq_fp32);
+/* RHF SCF of every molecule (pscf/scf.c:189-474 with a zero initial
+ * density): at most max_iters Fock builds; molecule m stops after the build
+ * whose electronic energy differs from the previous one by less than
+ * ediff.  energy[m] = final total energy, iters[m] = builds used (may be
+ * NULL), history[m * max_iters + it] = energy of build it (may be NULL). */
+int gtf_batch_scf(gtf_batch_t batch, int max_iters, double ediff, double *energy, int *iters, double *history);
+
 #ifdef __cplusplus
 }
 #endif
