@@ -77,6 +77,87 @@ def cpu_baseline(gbs, xyz, D, tol):
             "sample": "one full FP64 Fock build of the same molecule and density (%.1f s)" % dt}
 
 
+def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters):
+    """Many molecules per build: the seeded alkane / water-cluster batch.
+    Molecule-builds per second of (a) one PFock per molecule, FP64, built one
+    after another, (b) the batch, FP64, one HIP graph per build, (c) the
+    batch with FP32 ERIs below the Schwarz bound tau.  The density of each
+    molecule is its first SCF density.  With scf_iters > 0 the batched SCF is
+    run to convergence in FP64 and in mixed precision and the largest
+    total-energy difference is reported."""
+    from gtfock_amd import geometry
+    tmp = tempfile.mkdtemp()
+    mols = geometry.batch_molecules(nmol, seed)
+    bases = []
+    for k, (name, atoms) in enumerate(mols):
+        p = os.path.join(tmp, "m%03d.xyz" % k)
+        with open(p, "w") as fh:
+            fh.write(geometry.to_xyz(atoms))
+        bases.append(g.BasisSet(gbs, p))
+    nbf = [b.nbf for b in bases]
+    t0 = time.perf_counter()
+    bt = g.Batch(bases)
+    t_create = time.perf_counter() - t0
+    bt.scf(max_iters=2, ediff=0.0)  # leaves the first SCF density on the device
+    Ds = bt.get_density()
+    # (b) batched FP64, graph replays
+    bt.compute()
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bt.compute()
+    device_sync()
+    t_b64 = (time.perf_counter() - t0) / steps
+    G64 = bt.get_fock()
+    # (c) batched mixed
+    bt.set_precision(tau)
+    bt.compute()
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bt.compute()
+    device_sync()
+    t_b32 = (time.perf_counter() - t0) / steps
+    G32 = bt.get_fock()
+    st = bt.stats()
+    dF = max(float(np.abs(a - b).max()) for a, b in zip(G32, G64))
+    dE1 = max(float(abs(np.sum(D * (a - b)))) for D, a, b in zip(Ds, G32, G64))
+    bt.close()
+    # (a) one PFock per molecule, FP64, sequential
+    t_single = 0.0
+    for b, D in zip(bases, Ds):
+        pf = g.PFock(b)
+        pf.put_den_mat(D)
+        pf.commit()
+        pf.compute_fock()
+        device_sync()
+        t0 = time.perf_counter()
+        for _ in range(max(1, steps // 4)):
+            pf.compute_fock()
+        device_sync()
+        t_single += (time.perf_counter() - t0) / max(1, steps // 4)
+        pf.close()
+    out = {"molecules": nmol, "seed": seed, "nbf_min": min(nbf), "nbf_max": max(nbf),
+           "nbf_mean": float(np.mean(nbf)), "create_s": round(t_create, 3),
+           "unbatched_fp64": nmol / t_single, "batched_fp64": nmol / t_b64, "batched_mixed": nmol / t_b32,
+           "unit": "molecule Fock builds/s", "tau": tau,
+           "mixed_vs_unbatched_fp64": t_single / t_b32, "batched_fp64_vs_unbatched": t_single / t_b64,
+           "fp32_quartet_fraction": st["nq_fp32"] / max(1.0, st["nquartets"]),
+           "max_abs_dF": dF, "max_first_order_dE_Eh": dE1}
+    if scf_iters > 0:
+        b64 = g.Batch(bases)
+        e64, it64 = b64.scf(max_iters=scf_iters, ediff=1e-8)
+        b64.close()
+        b32 = g.Batch(bases, tau=tau)
+        t0 = time.perf_counter()
+        e32, it32 = b32.scf(max_iters=scf_iters, ediff=1e-8)
+        t_scf = time.perf_counter() - t0
+        b32.close()
+        out.update({"scf_max_abs_dE_Eh": float(np.abs(e32 - e64).max()),
+                    "scf_builds_mean": float(np.mean(it32)), "scf_mixed_s": t_scf})
+    return out
+
+
 def load_traffic(workload, cls_name):
     path = os.path.join(ROOT, "profiles", "eri_traffic_%s.json" % workload)
     if not os.path.exists(path):
@@ -98,6 +179,9 @@ def main():
     ap.add_argument("--tau", type=float, default=1e-4, help="Schwarz-bound threshold of the FP32 ERIs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--classes", action="store_true", help="print the per-class breakdown to stderr")
+    ap.add_argument("--batch", type=int, default=32, help="molecules of the batched leg (0: skip)")
+    ap.add_argument("--batch-seed", type=int, default=20261202)
+    ap.add_argument("--batch-scf", type=int, default=0, help="SCF iterations of the batched energy check")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -181,6 +265,10 @@ def main():
              "dE_Eh": float(abs(np.sum(D * (F32 - F64))))}
     pf.set_precision(0.0)
 
+    batched = None
+    if args.batch > 0:
+        batched = batch_leg(g, gbs, args.batch, args.batch_seed, args.tau, max(2, args.steps // 4), args.batch_scf)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(gbs, xyz, D, args.tol)
@@ -196,6 +284,7 @@ def main():
                    "integrals": int(st["uitl"]), "create_s": round(t_create, 3)},
         "roofline": roofline,
         "mixed": mixed,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     pf.close()

@@ -136,6 +136,14 @@ def lib():
         "gtf_batch_stats": ([_vp, _dp], ctypes.c_int),
         "gtf_batch_class_work": ([_vp, _lp, _dp, _lp, _dp], ctypes.c_int),
         "gtf_batch_scf": ([_vp, ctypes.c_int, ctypes.c_double, _dp, _ip, _dp], ctypes.c_int),
+        "gtf_batch_done_times": ([_vp, _dp], ctypes.c_int),
+        "gtf_server_create": ([ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_int, ctypes.c_double, ctypes.POINTER(_vp)], ctypes.c_int),
+        "gtf_server_submit": ([_vp, _vp], ctypes.c_long),
+        "gtf_server_poll": ([_vp, ctypes.c_long, _dp], ctypes.c_int),
+        "gtf_server_wait": ([_vp], ctypes.c_int),
+        "gtf_server_stats": ([_vp, _dp], ctypes.c_int),
+        "gtf_server_destroy": ([_vp], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -505,6 +513,53 @@ class Batch:
         if history:
             return e, it, hist.reshape(self.nmol, max_iters)
         return e, it
+
+
+class Server:
+    """SCF request server (gtf_server_*): jobs admitted by a device-memory
+    budget into cohorts that run as batches, several cohorts at once."""
+
+    JOB_KEYS = ["energy", "iters", "t_submit", "t_start", "t_done", "cohort", "mem_est"]
+
+    def __init__(self, hbm_budget_gb=64.0, max_cohorts=4, max_jobs_per_cohort=64, tolscr=1e-11, tau=0.0,
+                 max_iters=30, ediff=1e-8):
+        self.h = _vp()
+        st = lib().gtf_server_create(hbm_budget_gb * 2.0 ** 30, max_cohorts, max_jobs_per_cohort, tolscr, tau,
+                                     max_iters, ediff, ctypes.byref(self.h))
+        if st != 0:
+            self.h = None
+            raise GtfError("gtf_server_create failed (%d)" % st)
+        self._keep = {}
+
+    def submit(self, basis):
+        jid = int(lib().gtf_server_submit(self.h, basis.h))
+        if jid < 0:
+            raise GtfError("gtf_server_submit failed (%d)" % jid)
+        self._keep[jid] = basis  # the basis must outlive the job
+        return jid
+
+    def poll(self, jid):
+        out = np.zeros(7)
+        st = lib().gtf_server_poll(self.h, jid, _p(out))
+        d = dict(zip(self.JOB_KEYS, out.tolist()))
+        d["status"] = st
+        return d
+
+    def wait(self):
+        lib().gtf_server_wait(self.h)
+
+    def stats(self):
+        out = np.zeros(8)
+        lib().gtf_server_stats(self.h, _p(out))
+        return dict(zip(["jobs", "queued", "running", "cohorts", "peak_running", "bytes_in_use", "peak_bytes",
+                         "budget"], out.tolist()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gtf_server_destroy(self.h)
+            self.h = None
+
+    __del__ = close
 
 
 class PurifT(ctypes.Structure):

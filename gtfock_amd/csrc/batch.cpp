@@ -75,6 +75,7 @@ struct GtfBatch {
     std::vector<int> len_diis, bmax_id;
     std::vector<double> bmax, b_mat;
     std::vector<int> purif_iters;
+    std::vector<double> done_time;  // wall clock at which each molecule's SCF ended
     double time_fock = 0, time_diis = 0, time_purif = 0;
 
     BatchLayout layout() const { return BatchLayout{nmol, maxn, maxpld, d_off, d_n, d_poff, d_pld}; }
@@ -105,7 +106,7 @@ __global__ void bstore_slot_kernel(const long* __restrict__ poff, const int* __r
                                    const double* __restrict__ src, double* __restrict__ vecs, long stride)
 {
     const int m = blockIdx.y;
-    if (skip[m]) return;
+    if (skip[m] || slot[m] < 0 || slot[m] >= B_MAX_DIIS) return;
     const long nn = (long)pld[m] * pld[m];
     double* dst = vecs + (long)slot[m] * stride + poff[m];
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < nn; k += (long)gridDim.x * blockDim.x)
@@ -303,14 +304,21 @@ int GtfBatch::diis(int iter, const std::vector<int>& active)
         for (int m = 0; m < nmol; m++) {
             if (!active[m]) continue;
             slot[m] = len_diis[m] < B_MAX_DIIS ? len_diis[m]++ : bmax_id[m];
+            if (slot[m] < 0 || slot[m] >= B_MAX_DIIS) {
+                // only a non-finite B matrix leaves no largest error; such a
+                // molecule has already been stopped by the energy check
+                GTF_PRINTF("molecule %d: no DIIS slot (bmax_id %d)\n", m, bmax_id[m]);
+                return -1;
+            }
             nvec[m] = len_diis[m];
         }
         B_CHECK(hipMemcpyAsync(d_slot, slot.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, stream));
         B_CHECK(hipMemcpyAsync(d_nvec, nvec.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, stream));
         // e = X (F D S - S D F) X with the A * B^T convention of pdgemm3D_2
-        if (gtf_bgemm(stream, L, skip, p_W2, p_D, p_W, 1) || gtf_bgemm(stream, L, skip, p_W, p_S, p_D2, 1) ||
-            gtf_bantisym(stream, L, skip, p_D2, p_D3) || gtf_bgemm(stream, L, skip, p_X, p_D3, p_W, 1) ||
-            gtf_bgemm(stream, L, skip, p_W, p_X, p_D2, 1))
+        if (gtf_bgemm(stream, L, skip, nullptr, p_W2, p_D, p_W, 1) ||
+            gtf_bgemm(stream, L, skip, nullptr, p_W, p_S, p_D2, 1) || gtf_bantisym(stream, L, skip, p_D2, p_D3) ||
+            gtf_bgemm(stream, L, skip, nullptr, p_X, p_D3, p_W, 1) ||
+            gtf_bgemm(stream, L, skip, nullptr, p_W, p_X, p_D2, 1))
             return -1;
         bstore_slot_kernel<<<dim3(256, nmol), 256, 0, stream>>>(d_poff, d_pld, skip, d_slot, p_D2, evecs, ptotal);
         B_CHECK(hipGetLastError());
@@ -335,7 +343,9 @@ int GtfBatch::diis(int iter, const std::vector<int>& active)
         }
     }
     // F_orth = X F X
-    if (gtf_bgemm(stream, L, skip, p_X, p_W2, p_W, 1) || gtf_bgemm(stream, L, skip, p_W, p_X, p_F, 1)) return -1;
+    if (gtf_bgemm(stream, L, skip, nullptr, p_X, p_W2, p_W, 1) ||
+        gtf_bgemm(stream, L, skip, nullptr, p_W, p_X, p_F, 1))
+        return -1;
     if (iter > 1) {
         bstore_slot_kernel<<<dim3(256, nmol), 256, 0, stream>>>(d_poff, d_pld, skip, d_slot, p_F, Fvecs, ptotal);
         B_CHECK(hipGetLastError());
@@ -365,7 +375,8 @@ int GtfBatch::purify(const std::vector<int>& active)
     int it = 0;
     for (it = 0; it < 200; it++) {
         // molecules already converged skip their GEMMs through d_iters >= 0
-        if (gtf_bgemm(stream, L, d_iters, p_D, p_D, p_D2, 0) || gtf_bgemm(stream, L, d_iters, p_D, p_D2, p_D3, 0))
+        if (gtf_bgemm(stream, L, skip, d_iters, p_D, p_D, p_D2, 0) ||
+            gtf_bgemm(stream, L, skip, d_iters, p_D, p_D2, p_D3, 0))
             return -1;
         if (gtf_bpurif_stats(stream, L, skip, p_D, p_D2, p_D3, d_iters, d_stats, it, 1e-11)) return -1;
         if (gtf_bpurif_update(stream, L, skip, d_iters, d_stats, p_D, p_D2, p_D3, it)) return -1;
@@ -383,7 +394,9 @@ int GtfBatch::purify(const std::vector<int>& active)
     for (int m = 0; m < nmol; m++)
         if (active[m]) purif_iters[m] = it_h[m] < 0 ? 200 : it_h[m];
     // D_AO = X D X
-    if (gtf_bgemm(stream, L, skip, p_X, p_D, p_W, 1) || gtf_bgemm(stream, L, skip, p_W, p_X, p_D2, 1)) return -1;
+    if (gtf_bgemm(stream, L, skip, nullptr, p_X, p_D, p_W, 1) ||
+        gtf_bgemm(stream, L, skip, nullptr, p_W, p_X, p_D2, 1))
+        return -1;
     return gtf_bunpad(stream, L, skip, p_D2, d_D);
 }
 
@@ -419,6 +432,7 @@ int gtf_batch_create(BasisSet_t* basis, int nmol, double tolscr, gtf_batch_t* ou
     b->bmax_id.assign(nmol, -1);
     b->bmax.assign(nmol, DBL_MIN);
     b->purif_iters.assign(nmol, 0);
+    b->done_time.assign(nmol, 0.0);
     if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
         hipEventCreate(&b->ev1) != hipSuccess) {
         delete b;
@@ -590,9 +604,17 @@ int gtf_batch_scf(gtf_batch_t b, int max_iters, double ediff, double* energy, in
             if (!active[m]) continue;
             energy[m] = e[m] + b->enuc[m];
             if (history) history[(size_t)m * max_iters + it] = energy[m];
+            if (!std::isfinite(e[m])) {
+                // diverged: stop it, report iters = -(builds)
+                active[m] = 0;
+                nit[m] = -(it + 1);
+                b->done_time[m] = bwtime();
+                continue;
+            }
             if (it > 0 && fabs(e[m] - e0[m]) < ediff) {
                 active[m] = 0;
                 nit[m] = it + 1;
+                b->done_time[m] = bwtime();
             } else {
                 e0[m] = e[m];
                 nactive++;
@@ -610,9 +632,19 @@ int gtf_batch_scf(gtf_batch_t b, int max_iters, double ediff, double* energy, in
         B_CHECK(hipStreamSynchronize(b->stream));
         b->time_purif += bwtime() - t1;
     }
+    const double tend = bwtime();
+    for (int m = 0; m < nmol; m++)
+        if (active[m]) b->done_time[m] = tend;
     if (iters)
         for (int m = 0; m < nmol; m++) iters[m] = nit[m];
     return 0;
+}
+
+int gtf_batch_done_times(gtf_batch_t b, double* t)
+{
+    if (!b || !t) return -3;
+    for (int m = 0; m < b->nmol; m++) t[m] = b->done_time[m];
+    return b->nmol;
 }
 
 int gtf_batch_get_density(gtf_batch_t b, int m, double* D)

@@ -18,8 +18,9 @@ struct BatchLayout {
     const int* pld;
 };
 
-int gtf_bgemm(hipStream_t st, const BatchLayout& L, const int* skip, const double* A, const double* B, double* C,
-              int transB);
+// skip[m] != 0 or done[m] >= 0 (either array may be null) leaves molecule m alone
+int gtf_bgemm(hipStream_t st, const BatchLayout& L, const int* skip, const int* done, const double* A,
+              const double* B, double* C, int transB);
 int gtf_bpad(hipStream_t st, const BatchLayout& L, const int* skip, const double* packed, double* padded);
 int gtf_bunpad(hipStream_t st, const BatchLayout& L, const int* skip, const double* padded, double* packed);
 int gtf_bpurif_init(hipStream_t st, const BatchLayout& L, const int* nocc, const int* skip, const double* F, double* D);

@@ -18,15 +18,18 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 
 // C_m = A_m * B_m (TRANSB = 0) or A_m * B_m^T (TRANSB = 1) on padded blocks;
 // 64 x 64 C tile per workgroup as 2 x 2 v_mfma_f64_16x16x4_f64 tiles per wave
+// A molecule is skipped when skip[m] != 0 or, during purification, when it
+// has already converged (done[m] >= 0).
 template <int TRANSB>
 __global__ __launch_bounds__(256) void bgemm_kernel(const long* __restrict__ poff, const int* __restrict__ pld,
-                                                    const int* __restrict__ skip, const double* __restrict__ A0,
-                                                    const double* __restrict__ B0, double* __restrict__ C0)
+                                                    const int* __restrict__ skip, const int* __restrict__ done,
+                                                    const double* __restrict__ A0, const double* __restrict__ B0,
+                                                    double* __restrict__ C0)
 {
     const int m = blockIdx.z;
     const int ld = pld[m];
     const int row0 = blockIdx.y * BGT, col0 = blockIdx.x * BGT;
-    if (row0 >= ld || col0 >= ld || (skip && skip[m])) return;
+    if (row0 >= ld || col0 >= ld || (skip && skip[m]) || (done && done[m] >= 0)) return;
     const double* A = A0 + poff[m];
     const double* B = B0 + poff[m];
     double* C = C0 + poff[m];
@@ -301,13 +304,13 @@ static dim3 per_mol_grid(int maxelems, int nmol)
     return dim3((unsigned)std::max(1, std::min((maxelems + 255) / 256, 256)), (unsigned)nmol);
 }
 
-int gtf_bgemm(hipStream_t st, const BatchLayout& L, const int* skip, const double* A, const double* B, double* C,
-              int transB)
+int gtf_bgemm(hipStream_t st, const BatchLayout& L, const int* skip, const int* done, const double* A,
+              const double* B, double* C, int transB)
 {
     if (L.nmol <= 0) return 0;
     dim3 grid(L.maxpld / BGT, L.maxpld / BGT, L.nmol);
-    if (transB) bgemm_kernel<1><<<grid, 256, 0, st>>>(L.poff, L.pld, skip, A, B, C);
-    else bgemm_kernel<0><<<grid, 256, 0, st>>>(L.poff, L.pld, skip, A, B, C);
+    if (transB) bgemm_kernel<1><<<grid, 256, 0, st>>>(L.poff, L.pld, skip, done, A, B, C);
+    else bgemm_kernel<0><<<grid, 256, 0, st>>>(L.poff, L.pld, skip, done, A, B, C);
     GTF_CHECK(hipGetLastError());
     return 0;
 }

@@ -258,6 +258,10 @@ static int diis(purif_t* p, double* D_block, double* F_block, int iter)
     double* curF = d->F;
     if (iter > 1) {
         const int cur = p->len_diis < MAX_DIIS ? p->len_diis++ : p->bmax_id;
+        if (cur < 0 || cur >= MAX_DIIS) {
+            fprintf(stderr, "compute_diis: no DIIS slot (non-finite error vectors)\n");
+            return -1;
+        }
         double* e = d->evecs + cur * m;
         curF = d->Fvecs + cur * m;
         // e = X (FDS - (FDS)^T)^T X^T with A*B^T products (pscf/purif.c:524-565)

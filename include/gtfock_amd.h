@@ -112,6 +112,34 @@ int gtf_batch_class_work(gtf_batch_t batch, long *nquartets, double *nprimquarte
  * ediff.  energy[m] = final total energy, iters[m] = builds used (may be
  * NULL), history[m * max_iters + it] = energy of build it (may be NULL). */
 int gtf_batch_scf(gtf_batch_t batch, int max_iters, double ediff, double *energy, int *iters, double *history);
+/* wall-clock time (gettimeofday seconds) at which each molecule's last
+ * gtf_batch_scf ended; returns nmol */
+int gtf_batch_done_times(gtf_batch_t batch, double *t);
+
+/* ---- an SCF request server --------------------------------------------
+ *
+ * Jobs queue on submission; a scheduler admits them in arrival order into
+ * cohorts that fit the device-memory budget (footprint estimated from the
+ * basis), at most max_jobs_per_cohort each, and runs every cohort as one
+ * batch (gtf_batch_scf) on its own thread and streams, up to max_cohorts at
+ * once.  The BasisSet of a job must stay alive until the job is done. */
+typedef struct GtfServer *gtf_server_t;
+
+int gtf_server_create(double hbm_budget_bytes, int max_cohorts, int max_jobs_per_cohort, double tolscr, double tau,
+                      int max_iters, double ediff, gtf_server_t *server);
+/* returns the job id (>= 0) or -3 */
+long gtf_server_submit(gtf_server_t server, BasisSet_t basis);
+/* status: 0 queued, 1 running, 2 done, < 0 failed (-2: larger than the
+ * budget, -4: cancelled); out[7] = {energy, builds, submit time, start
+ * time, done time, cohort, estimated bytes} (out may be NULL) */
+int gtf_server_poll(gtf_server_t server, long job, double *out);
+/* block until no job is queued or running */
+int gtf_server_wait(gtf_server_t server);
+/* out[8] = {jobs, queued, running cohorts, cohorts started, peak running
+ *           cohorts, bytes in use, peak bytes, budget} */
+int gtf_server_stats(gtf_server_t server, double *out);
+/* cancels queued jobs, waits for running cohorts */
+void gtf_server_destroy(gtf_server_t server);
 
 #ifdef __cplusplus
 }

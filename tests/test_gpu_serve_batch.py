@@ -141,3 +141,55 @@ def test_batch_mixed_energy_error(g, gbs, small_batch):
     assert np.abs(e64 - e32).max() < 1e-6
     b64.close()
     b32.close()
+
+
+@gpu
+def test_server_alone_equals_in_mix(g, gbs, small_batch):
+    """A job's energy is the same when it runs alone and when it shares a
+    cohort (and the device) with other jobs."""
+    bases = [g.BasisSet(gbs, p) for p in small_batch]
+    # alone: one job per cohort, one cohort at a time
+    s1 = g.Server(hbm_budget_gb=8.0, max_cohorts=1, max_jobs_per_cohort=1, max_iters=20, ediff=1e-10)
+    ids = [s1.submit(b) for b in bases]
+    s1.wait()
+    alone = [s1.poll(i) for i in ids]
+    s1.close()
+    # in the mix: all jobs together, several cohorts overlapping
+    s2 = g.Server(hbm_budget_gb=8.0, max_cohorts=3, max_jobs_per_cohort=3, max_iters=20, ediff=1e-10)
+    ids = [s2.submit(b) for b in bases + bases]
+    s2.wait()
+    mix = [s2.poll(i) for i in ids]
+    st = s2.stats()
+    s2.close()
+    assert all(r["status"] == 2 for r in alone + mix)
+    assert st["cohorts"] >= 2
+    for k, r in enumerate(mix):
+        a = alone[k % len(bases)]
+        print("job %d: alone %.12f (%d builds), mix %.12f (%d builds, cohort %d)" % (
+            k, a["energy"], a["iters"], r["energy"], r["iters"], r["cohort"]))
+        assert abs(r["energy"] - a["energy"]) < 1e-9
+        assert r["iters"] == a["iters"]
+
+
+@gpu
+def test_server_budget_admission(g, gbs, small_batch):
+    """Under a budget that holds little more than the largest job every job
+    still completes, the budget is never exceeded, and a job larger than the
+    whole budget fails instead of blocking the queue."""
+    bases = [g.BasisSet(gbs, p) for p in small_batch[:3]]
+    big = g.BasisSet(gbs, small_batch[4])
+    s = g.Server(hbm_budget_gb=8.0, max_cohorts=1, max_jobs_per_cohort=8, max_iters=2, ediff=1e-9)
+    est = [s.poll(s.submit(b))["mem_est"] for b in bases + [big]]
+    s.wait()
+    s.close()
+    budget = 1.05 * max(est[:3])
+    assert est[3] > budget
+    s = g.Server(hbm_budget_gb=budget / 2 ** 30, max_cohorts=4, max_jobs_per_cohort=8, max_iters=15, ediff=1e-9)
+    ids = [s.submit(b) for b in bases] + [s.submit(big)] + [s.submit(b) for b in bases]
+    s.wait()
+    res = [s.poll(i) for i in ids]
+    st = s.stats()
+    s.close()
+    print([r["status"] for r in res], st)
+    assert st["peak_bytes"] <= st["budget"]
+    assert [r["status"] for r in res] == [2, 2, 2, -2, 2, 2, 2]
