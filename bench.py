@@ -236,7 +236,7 @@ def main():
     achieved = dom_fl / dom_s
     roofline = {"bound": "mfma", "peak_kind": "FP64 (vector and matrix peaks are both 78.6 TF/s on MI355X; "
                 "the ERI kernels run on the FP64 VALU)",
-                "kernel": "eri_kernel<%s, EPI_JK>" % names[dom],
+                "kernel": pf.class_kernel(dom), "class": names[dom],
                 "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK, "traffic": load_traffic(args.workload, names[dom]),
                 "algorithmic_flops_per_launch": dom_fl, "avg_launch_ms": dom_s * 1e3,

@@ -108,6 +108,7 @@ def lib():
         "gtf_pfock_device_ptrs": ([_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                    ctypes.POINTER(_vp)], ctypes.c_int),
         "gtf_pfock_stream": ([_vp], _vp),
+        "gtf_pfock_class_info": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
         "gtf_pfock_class_work": ([_vp, _lp, _dp], ctypes.c_int),
         "gtf_pfock_set_timing": ([_vp, ctypes.c_int], ctypes.c_int),
         "gtf_pfock_set_precision": ([_vp, ctypes.c_double], ctypes.c_int),
@@ -314,6 +315,15 @@ class PFock:
 
     def set_timing(self, on=True):
         _check(lib().gtf_pfock_set_timing(self.h, 1 if on else 0), "gtf_pfock_set_timing")
+
+    def class_kernel(self, cls):
+        """Name of the kernel that runs quartet class cls (FP64 part)."""
+        info = np.zeros(8, dtype=np.int32)
+        _check(lib().gtf_pfock_class_info(self.h, cls, info.ctypes.data_as(_ip)), "gtf_pfock_class_info")
+        la, lb, lc, ld, nc = (int(x) for x in info[:5])
+        if info[6] > 0:
+            return "eri_jk_tile_kernel<%d, %d, %d, %d, %d, double>" % (la, lb, lc, ld, nc)
+        return "eri_kernel<%d, %d, %d, %d, %d, 0>" % (la, lb, lc, ld, nc)
 
     def class_times(self):
         ms = np.zeros(21)

@@ -424,7 +424,9 @@ int GtfSystem::build_tiles()
         gtf_class_types(cls, A, B);
         const EriClassInfo* ci = gtf_eri_class(cls);
         const int NA = 2 * ci->la + 1, NB = 2 * ci->lb + 1;
-        tile_lds[cls] = sizeof(double) * ((size_t)(NA + NB) * nbf + NA * NB);
+        const int NCD = (2 * ci->lc + 1) * (2 * ci->ld + 1);
+        tile_lds[cls] = sizeof(double) * ((size_t)(NA + NB) * nbf + NA * NB +
+                                          (ci->nsplit > 1 ? (size_t)tile_kets(ci->nsplit) * NCD : 0));
         ntiles[cls] = ntiles_lo[cls] = 0;
         nq_lo[cls] = 0;
         nprimq_lo[cls] = 0.0;
@@ -434,7 +436,7 @@ int GtfSystem::build_tiles()
         if (nq[cls] == 0 || !use_tiles || tile_lds[cls] > max_lds) continue;
         const std::vector<double>& vs = h_sorted[B];
         const std::vector<double> kp = ket_prim_prefix(B);
-        const int tq = std::max(1, GTF_TILE_UNITS / ci->nsplit);
+        const int tq = tile_kets(ci->nsplit);
         std::vector<int4> hi, lo;
         for (int r = 0; r < nsig[A]; r++) {
             const long c = h_exact[cls][r];

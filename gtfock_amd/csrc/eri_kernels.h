@@ -82,7 +82,7 @@ __device__ __forceinline__ int find_row(const long* __restrict__ off, int nrows,
 template <int LA, int LB, int LC, int LD, int NC, bool STRIP>
 __device__ __forceinline__ void digest(const EriArgs& a, int bra, int ket, int c0,
                                        const double (&acc)[NC][EriShape<LA, LB, LC, LD>::NCD], double* jmn,
-                                       double* strip, int nbf, int unused);
+                                       double* strip, int nbf, double* jpq);
 
 // the Boys-grid columns L..L+6 of one class staged in LDS (40 KB)
 #ifndef GTF_BOYS_LDS
@@ -132,7 +132,7 @@ __device__ __forceinline__ void eri_unit(const EriArgs& a, long u, const double*
         });
         atomicMax(a.vbits + bra, (unsigned long long)__double_as_longlong(mx));
     } else {
-        digest<LA, LB, LC, LD, NC, false>(a, bra, ket, c0, acc, nullptr, nullptr, 0, 0);
+        digest<LA, LB, LC, LD, NC, false>(a, bra, ket, c0, acc, nullptr, nullptr, 0, nullptr);
     }
 }
 
@@ -141,13 +141,13 @@ __device__ __forceinline__ void eri_unit(const EriArgs& a, long u, const double*
 // STRIP = true:  the bra-side blocks (J_MN and the exchange blocks K_MP, K_NP,
 // K_MQ, K_NQ, which all lie in the rows of shells M and N) go to LDS
 // (jmn[NA*NB], strip[(NA+NB) * nbf], row r < NA is function fM + r, row
-// NA + r is fN + r); only J_PQ is added to HBM.
+// NA + r is fN + r); J_PQ goes to the ket's LDS row jpq[NCD] when one is
+// given (the threads sharing a quartet then meet in LDS), else to HBM.
 template <int LA, int LB, int LC, int LD, int NC, bool STRIP>
 __device__ __forceinline__ void digest(const EriArgs& a, int bra, int ket, int c0,
                                        const double (&acc)[NC][EriShape<LA, LB, LC, LD>::NCD], double* jmn,
-                                       double* strip, int nbf, int unused)
+                                       double* strip, int nbf, double* jpq)
 {
-    (void)unused;
     using S = EriShape<LA, LB, LC, LD>;
     constexpr int NCD = S::NCD, ND = S::ND, NB = S::NB, NA = S::NA;
     const int M = a.pv.s1[bra], N = a.pv.s2[bra], P = a.pv.s1[ket], Q = a.pv.s2[ket];
@@ -189,7 +189,8 @@ __device__ __forceinline__ void digest(const EriArgs& a, int bra, int ket, int c
                 constexpr int c = decltype(c_)::value;
                 s += dmn[c] * acc[c][cd];
             });
-            GTF_ATOMIC_ADD(&J[(long)(oP + cd / ND) * ld + oQ + cd % ND], cJ2 * s);
+            if (jpq) atomicAdd(&jpq[cd], cJ2 * s);
+            else GTF_ATOMIC_ADD(&J[(long)(oP + cd / ND) * ld + oQ + cd % ND], cJ2 * s);
         });
     }
     // K_MP -= cMP sum_q D_nq I,  K_NP -= cNP sum_q D_mq I
@@ -275,9 +276,15 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
 // Fused ERI + J/K digestion over bra-row tiles.  A tile is up to
 // GTF_TILE_UNITS work units of one bra row; the workgroup accumulates J_MN and
 // the exchange blocks of the bra shells in LDS and flushes them with one HBM
-// atomic per touched element when the tile is done.  Dynamic LDS:
-// (NA + NB) * nbf + NA * NB doubles, nbf = the largest molecule's size.
+// atomic per touched element when the tile is done.  When several threads
+// share a quartet (NSPLIT > 1) the J_PQ block of every ket of the tile is
+// also summed in LDS first.  Dynamic LDS: (NA + NB) * nbf + NA * NB doubles,
+// nbf = the largest molecule's size, plus tile_kets * NCD doubles when
+// NSPLIT > 1.
 #define GTF_TILE_UNITS 1024
+
+// kets per tile of a class with nsplit threads per quartet
+__host__ __device__ constexpr int tile_kets(int nsplit) { return GTF_TILE_UNITS / nsplit > 0 ? GTF_TILE_UNITS / nsplit : 1; }
 
 template <int LA, int LB, int LC, int LD, int NC, typename Real>
 __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
@@ -286,6 +293,8 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
     constexpr int NSPLIT = S::NAB / NC;
     constexpr int NA = S::NA, NB = S::NB, NCD = S::NCD;
     constexpr bool LOW = std::is_same<Real, float>::value;
+    constexpr bool JPQ_LDS = NSPLIT > 1;
+    constexpr int ND = S::ND;
     extern __shared__ double smem[];
     __shared__ int s_task;
     for (;;) {
@@ -300,7 +309,9 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
         const int nstrip = (NA + NB) * nbf;
         double* strip = smem;
         double* jmn = smem + nstrip;
-        for (int k = threadIdx.x; k < nstrip + NA * NB; k += 256) smem[k] = 0.0;
+        double* jpq = jmn + NA * NB;
+        const int njpq = JPQ_LDS ? (t.z - t.y) * NCD : 0;
+        for (int k = threadIdx.x; k < nstrip + NA * NB + njpq; k += 256) smem[k] = 0.0;
         __syncthreads();
         const int nunits = (t.z - t.y) * NSPLIT;
         for (int u = threadIdx.x; u < nunits; u += 256) {
@@ -323,7 +334,8 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
             } else {
                 eri_block<LA, LB, LC, LD, NC, double, BOYS_NCOL>(a.pv, bra, ket, c0, a.boys, acc);
             }
-            digest<LA, LB, LC, LD, NC, true>(a, bra, ket, c0, acc, jmn, strip, nbf, 0);
+            digest<LA, LB, LC, LD, NC, true>(a, bra, ket, c0, acc, jmn, strip, nbf,
+                                             JPQ_LDS ? jpq + q * NCD : nullptr);
         }
         __syncthreads();
         const int M = a.pv.s1[bra], N = a.pv.s2[bra];
@@ -341,6 +353,17 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
         for (int k = threadIdx.x; k < NA * NB; k += 256) {
             const double v = jmn[k];
             if (v != 0.0) GTF_ATOMIC_ADD(&a.J[base + (long)(oM + k / NB) * ld + oN + k % NB], v);
+        }
+        if constexpr (JPQ_LDS) {
+            for (int k = threadIdx.x; k < njpq; k += 256) {
+                const double v = jpq[k];
+                if (v != 0.0) {
+                    const int q = k / NCD, cd = k - q * NCD;
+                    const int ket = a.ket_list[t.y + q];
+                    const int oP = a.fstart[a.pv.s1[ket]], oQ = a.fstart[a.pv.s2[ket]];
+                    GTF_ATOMIC_ADD(&a.J[base + (long)(oP + cd / ND) * ld + oQ + cd % ND], v);
+                }
+            }
         }
         __syncthreads();
     }

@@ -547,6 +547,21 @@ int gtf_pfock_device_ptrs(PFock_t pf, int index, double** D, double** J, double*
 
 void* gtf_pfock_stream(PFock_t pf) { return (void*)pf->stream; }
 
+int gtf_pfock_class_info(PFock_t pf, int cls, int* info)
+{
+    if (!pf || !info || cls < 0 || cls >= GTF_NCLASSES) return -1;
+    const EriClassInfo* ci = gtf_eri_class(cls);
+    info[0] = ci->la;
+    info[1] = ci->lb;
+    info[2] = ci->lc;
+    info[3] = ci->ld;
+    info[4] = ci->nc;
+    info[5] = ci->nsplit;
+    info[6] = pf->sys.ntiles[cls];
+    info[7] = pf->sys.ntiles_lo[cls];
+    return 0;
+}
+
 int gtf_pfock_class_work(PFock_t pf, long* nquartets, double* nprimquartets)
 {
     for (int c = 0; c < GTF_NCLASSES; c++) {

@@ -54,6 +54,12 @@ int gtf_pfock_set_timing(PFock_t pfock, int on);
 /* Accumulated milliseconds and launch counts per class since enabling. */
 int gtf_pfock_class_times(PFock_t pfock, double *ms, long *launches);
 
+/* Kernel shape of quartet class cls: info[8] = {La, Lb, Lc, Ld, bra
+ * components per thread, threads per quartet, FP64 bra-row tiles, FP32
+ * bra-row tiles}; with no tiles the class runs eri_kernel<..., EPI_JK>,
+ * else eri_jk_tile_kernel<La, Lb, Lc, Ld, NC, double / float>. */
+int gtf_pfock_class_info(PFock_t pfock, int cls, int *info);
+
 /* The engine's HIP stream (hipStream_t). */
 void *gtf_pfock_stream(PFock_t pfock);
 
