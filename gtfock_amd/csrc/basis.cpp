@@ -23,6 +23,7 @@
 
 #include "../../include/CInt.h"
 #include "basis_internal.h"
+#include "gtf_internal.h"
 
 static const char* kElems[] = {"X",  "H",  "He", "Li", "Be", "B",  "C",  "N",  "O",  "F",
                                "Ne", "Na", "Mg", "Al", "Si", "P",  "S",  "Cl", "Ar", "K",
@@ -371,6 +372,26 @@ void CInt_getInitialGuess(BasisSet_t b, int atomid, double** guess, int* spos, i
 // Boys table: F_m(i * delta) for m = 0..ncol-1 from the series
 //   F_m(T) = exp(-T) sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1))
 // for the top order, then downward recursion, in long double.
+// the compact Taylor tables of the ERI kernels (boys_eval8): for order M and
+// grid point i, coefficient k is F_{M+k}(i * BOYS8_DELTA) / k!
+void gtf_boys_taylor_tables(double* t8, float* t8f)
+{
+    const int ncol = BOYS8_NM + BOYS8_NTERM - 1;
+    std::vector<double> F((size_t)BOYS8_NGRID * ncol);
+    gtf_boys_table(F.data(), BOYS8_NGRID, ncol, BOYS8_DELTA);
+    for (int M = 0; M < BOYS8_NM; M++)
+        for (int i = 0; i < BOYS8_NGRID; i++) {
+            double fact = 1.0;
+            for (int k = 0; k < BOYS8_NTERM; k++) {
+                if (k > 0) fact *= k;
+                const double c = F[(size_t)i * ncol + M + k] / fact;
+                const size_t at = (size_t)M * BOYS8_SLAB + (size_t)i * BOYS8_NTERM + k;
+                t8[at] = c;
+                t8f[at] = (float)c;
+            }
+        }
+}
+
 void gtf_boys_table(double* tab, int ngrid, int ncol, double delta)
 {
     for (int i = 0; i < ngrid; i++) {

@@ -111,6 +111,15 @@ int GtfSystem::init(const std::vector<const BasisSet*>& mols, hipStream_t st)
     std::vector<double> boys((size_t)BOYS_NGRID * BOYS_NCOL);
     gtf_boys_table(boys.data(), BOYS_NGRID, BOYS_NCOL, BOYS_DELTA);
     if (upload(d_boys, boys, st)) return -1;
+    std::vector<double> b8((size_t)BOYS8_NM * BOYS8_SLAB);
+    std::vector<float> b8f(b8.size());
+    gtf_boys_taylor_tables(b8.data(), b8f.data());
+    d_boys8 = dalloc<double>(b8.size());
+    d_boys8f = dalloc<float>(b8f.size());
+    if (!d_boys8 || !d_boys8f) return -2;
+    if (upload(d_boys8, b8, st) || upload(d_boys8f, b8f, st)) return -1;
+    // the uploads read host vectors that go out of scope here
+    SYS_CHECK(hipStreamSynchronize(st));
 
     // all unordered shell pairs (A >= B by local index) of every molecule,
     // canonical orientation by L
@@ -227,7 +236,8 @@ int GtfSystem::screen(double tolscr)
             a.qlist = d_q + off;
             a.pv = pv;
             a.fstart = d_fstart;
-            a.boys = d_boys;
+            a.boys8 = d_boys8;
+            a.boys8f = d_boys8f;
             a.vbits = d_vbits;
             if (launch(ci->diag, a, (int)((a.nunits + 255) / 256), stream)) return -1;
             off += n;
@@ -248,10 +258,11 @@ int GtfSystem::screen(double tolscr)
         mol_vmax[h_pair_mol[id]] = std::max(mol_vmax[h_pair_mol[id]], v);
     }
     vmax = mol_vmax[0];
-    // significant pairs of molecule m: v > tol^2 / vmax_m (pfock/screening.c:111,129);
-    // one segment per molecule in each type list
+    // significant pairs of molecule m: v > tol^2 / vmax_m (pfock/screening.c:111,129),
+    // in one segment per (molecule, primitive-count bucket)
     std::vector<double> eta(nmol);
     for (int m = 0; m < nmol; m++) eta[m] = mol_vmax[m] > 0.0 ? tol2 / mol_vmax[m] : HUGE_VAL;
+    nseg = nmol * NBUCKET;
     for (int t = 0; t < GTF_NPAIRTYPES; t++) {
         std::vector<int>& s = h_sig[t];
         s.clear();
@@ -259,85 +270,127 @@ int GtfSystem::screen(double tolscr)
         h_rank[t].clear();
         h_sigmol[t].clear();
         h_sorted[t].clear();
-        seg_start[t].assign(nmol, 0);
-        seg_len[t].assign(nmol, 0);
+        h_sortrank[t].clear();
+        h_sortnprim[t].clear();
+        seg_start[t].assign(nseg, 0);
+        seg_len[t].assign(nseg, 0);
         // bytype[t] is ordered by pair id, hence grouped by molecule
         size_t i0 = 0;
         for (int m = 0; m < nmol; m++) {
-            std::vector<int> seg;
+            std::vector<int> mol;
             while (i0 < bytype[t].size() && h_pair_mol[bytype[t][i0]] == m) {
                 const int id = bytype[t][i0++];
-                if (h_v[id] > eta[m]) seg.push_back(id);
+                if (h_v[id] > eta[m]) mol.push_back(id);
             }
-            std::sort(seg.begin(), seg.end(),
+            std::sort(mol.begin(), mol.end(),
                       [&](int x, int y) { return h_v[x] != h_v[y] ? h_v[x] > h_v[y] : x < y; });
-            // value rank, then each block re-sorted by primitive count (descending)
-            std::vector<int> order(seg.size());
-            std::iota(order.begin(), order.end(), 0);
-            for (size_t b0 = 0; b0 < seg.size(); b0 += GTF_KET_BLOCK) {
-                const size_t b1 = std::min(seg.size(), b0 + GTF_KET_BLOCK);
-                std::stable_sort(order.begin() + b0, order.begin() + b1,
-                                 [&](int x, int y) { return h_pnprim[seg[x]] > h_pnprim[seg[y]]; });
-            }
-            seg_start[t][m] = (int)s.size();
-            seg_len[t][m] = (int)seg.size();
-            for (size_t k = 0; k < seg.size(); k++) {
-                s.push_back(seg[order[k]]);
-                h_rank[t].push_back(order[k]);
-                h_sigv[t].push_back(h_v[seg[order[k]]]);
-                h_sigmol[t].push_back(m);
-                h_sorted[t].push_back(h_v[seg[k]]);
+            std::vector<int> rank_of(mol.size());
+            std::iota(rank_of.begin(), rank_of.end(), 0);
+            for (int b = 0; b < NBUCKET; b++) {
+                // the bucket's pairs in value order (ranks rising)
+                std::vector<int> seg, segrank;
+                for (size_t k = 0; k < mol.size(); k++)
+                    if (bucket(h_pnprim[mol[k]]) == b) {
+                        seg.push_back(mol[k]);
+                        segrank.push_back((int)k);
+                    }
+                // each block re-sorted by primitive count (descending)
+                std::vector<int> order(seg.size());
+                std::iota(order.begin(), order.end(), 0);
+                for (size_t b0 = 0; b0 < seg.size(); b0 += GTF_KET_BLOCK) {
+                    const size_t b1 = std::min(seg.size(), b0 + GTF_KET_BLOCK);
+                    std::stable_sort(order.begin() + b0, order.begin() + b1,
+                                     [&](int x, int y) { return h_pnprim[seg[x]] > h_pnprim[seg[y]]; });
+                }
+                const int sg = m * NBUCKET + b;
+                seg_start[t][sg] = (int)s.size();
+                seg_len[t][sg] = (int)seg.size();
+                for (size_t k = 0; k < seg.size(); k++) {
+                    s.push_back(seg[order[k]]);
+                    h_rank[t].push_back(segrank[order[k]]);
+                    h_sigv[t].push_back(h_v[seg[order[k]]]);
+                    h_sigmol[t].push_back(m);
+                    h_sorted[t].push_back(h_v[seg[k]]);
+                    h_sortrank[t].push_back(segrank[k]);
+                    h_sortnprim[t].push_back(h_pnprim[seg[k]]);
+                }
             }
         }
         nsig[t] = (int)s.size();
         d_sig[t] = dalloc<int>(s.size());
         d_sigv[t] = dalloc<double>(s.size());
         d_rank[t] = dalloc<int>(s.size());
-        d_sigmol[t] = dalloc<int>(s.size());
         d_sigv_sorted[t] = dalloc<double>(s.size());
-        d_seg_start[t] = dalloc<int>(nmol);
-        d_seg_len[t] = dalloc<int>(nmol);
-        if (!d_sig[t] || !d_sigv[t] || !d_rank[t] || !d_sigmol[t] || !d_sigv_sorted[t] || !d_seg_start[t] ||
+        d_sortrank[t] = dalloc<int>(s.size());
+        d_seg_start[t] = dalloc<int>(nseg);
+        d_seg_len[t] = dalloc<int>(nseg);
+        if (!d_sig[t] || !d_sigv[t] || !d_rank[t] || !d_sigv_sorted[t] || !d_sortrank[t] || !d_seg_start[t] ||
             !d_seg_len[t])
             return -2;
         if (upload(d_sig[t], s, stream) || upload(d_sigv[t], h_sigv[t], stream) ||
-            upload(d_rank[t], h_rank[t], stream) || upload(d_sigmol[t], h_sigmol[t], stream) ||
-            upload(d_sigv_sorted[t], h_sorted[t], stream) || upload(d_seg_start[t], seg_start[t], stream) ||
+            upload(d_rank[t], h_rank[t], stream) || upload(d_sigv_sorted[t], h_sorted[t], stream) ||
+            upload(d_sortrank[t], h_sortrank[t], stream) || upload(d_seg_start[t], seg_start[t], stream) ||
             upload(d_seg_len[t], seg_len[t], stream))
             return -1;
     }
-    // quartet rows per class: bra type A >= ket type B, ket in the bra's molecule
-    int maxn = 1;
-    for (int t = 0; t < GTF_NPAIRTYPES; t++) maxn = std::max(maxn, nsig[t]);
-    long* d_counts = dalloc<long>(maxn);
-    long* d_exact = dalloc<long>(maxn);
+    // quartet rows per class: bra type A >= ket type B; a row pairs one bra
+    // with one non-empty ket segment of its molecule
+    int maxrows = 1;
+    for (int cls = 0; cls < GTF_NCLASSES; cls++) {
+        int A, B;
+        gtf_class_types(cls, A, B);
+        h_row_bra[cls].clear();
+        h_row_seg[cls].clear();
+        for (int r = 0; r < nsig[A]; r++) {
+            const int m = h_sigmol[A][r];
+            for (int b = 0; b < NBUCKET; b++)
+                if (seg_len[B][m * NBUCKET + b] > 0) {
+                    h_row_bra[cls].push_back(r);
+                    h_row_seg[cls].push_back(m * NBUCKET + b);
+                }
+        }
+        nrows[cls] = (int)h_row_bra[cls].size();
+        maxrows = std::max(maxrows, nrows[cls]);
+    }
+    long* d_counts = dalloc<long>(maxrows);
+    long* d_exact = dalloc<long>(maxrows);
     if (!d_counts || !d_exact) return -2;
     for (int cls = 0; cls < GTF_NCLASSES; cls++) {
         int A, B;
         gtf_class_types(cls, A, B);
+        const int nr = nrows[cls];
         nq[cls] = 0;
         nslots[cls] = 0;
         nprimq[cls] = 0.0;
-        h_exact[cls].assign(nsig[A], 0);
-        d_rowoff[cls] = dalloc<long>(nsig[A] + 1);
-        if (!d_rowoff[cls]) return -2;
-        if (nsig[A] == 0 || nsig[B] == 0) continue;
-        if (gtf_count_rows(stream, nsig[A], d_sigv[A], d_rank[A], d_sigmol[A], d_sigv_sorted[B], d_seg_start[B],
-                           d_seg_len[B], tol2, A == B, d_exact, d_counts, d_rowoff[cls]))
+        h_exact[cls].assign(nr, 0);
+        std::vector<int> kstart(nr);
+        for (int k = 0; k < nr; k++) kstart[k] = seg_start[B][h_row_seg[cls][k]];
+        d_rowoff[cls] = dalloc<long>(nr + 1);
+        d_row_bra[cls] = dalloc<int>(nr);
+        d_row_seg[cls] = dalloc<int>(nr);
+        d_row_kstart[cls] = dalloc<int>(nr);
+        if (!d_rowoff[cls] || !d_row_bra[cls] || !d_row_seg[cls] || !d_row_kstart[cls]) return -2;
+        if (nr == 0) continue;
+        if (upload(d_row_bra[cls], h_row_bra[cls], stream) || upload(d_row_seg[cls], h_row_seg[cls], stream) ||
+            upload(d_row_kstart[cls], kstart, stream))
             return -1;
-        std::vector<long> off(nsig[A] + 1), exact(nsig[A]);
+        if (gtf_count_rows(stream, nr, d_row_bra[cls], d_row_seg[cls], d_sigv[A], d_rank[A], d_sigv_sorted[B],
+                           d_sortrank[B], d_seg_start[B], d_seg_len[B], tol2, A == B, d_exact, d_counts,
+                           d_rowoff[cls]))
+            return -1;
+        std::vector<long> off(nr + 1), exact(nr);
         SYS_CHECK(hipMemcpyAsync(off.data(), d_rowoff[cls], sizeof(long) * off.size(), hipMemcpyDeviceToHost, stream));
         SYS_CHECK(hipMemcpyAsync(exact.data(), d_exact, sizeof(long) * exact.size(), hipMemcpyDeviceToHost, stream));
         SYS_CHECK(hipStreamSynchronize(stream));
-        nslots[cls] = off[nsig[A]];
+        nslots[cls] = off[nr];
         // exact quartet and primitive-quartet counts (value order per segment)
         const std::vector<double> kp = ket_prim_prefix(B);
         double w = 0.0;
         long qn = 0;
-        for (int r = 0; r < nsig[A]; r++) {
-            const int s0 = seg_start[B][h_sigmol[A][r]];
-            w += (double)h_pnprim[h_sig[A][r]] * (kp[s0 + exact[r]] - kp[s0]);
-            qn += exact[r];
+        for (int k = 0; k < nr; k++) {
+            const int s0 = kstart[k];
+            w += (double)h_pnprim[h_sig[A][h_row_bra[cls][k]]] * (kp[s0 + exact[k]] - kp[s0]);
+            qn += exact[k];
         }
         nq[cls] = qn;
         nprimq[cls] = w;
@@ -354,10 +407,8 @@ int GtfSystem::screen(double tolscr)
 // (segment by segment): kp[s0 + k] - kp[s0] = sum over the k largest of a segment
 std::vector<double> GtfSystem::ket_prim_prefix(int B) const
 {
-    std::vector<int> byrank(nsig[B]);
-    for (int j = 0; j < nsig[B]; j++) byrank[seg_start[B][h_sigmol[B][j]] + h_rank[B][j]] = h_sig[B][j];
     std::vector<double> kp(nsig[B] + 1, 0.0);
-    for (int j = 0; j < nsig[B]; j++) kp[j + 1] = kp[j] + h_pnprim[byrank[j]];
+    for (int j = 0; j < nsig[B]; j++) kp[j + 1] = kp[j] + h_sortnprim[B][j];
     return kp;
 }
 
@@ -368,10 +419,10 @@ long GtfSystem::quartets(int* out, long cap) const
         if (nq[cls] == 0) continue;
         int A, B;
         gtf_class_types(cls, A, B);
-        for (int r = 0; r < nsig[A]; r++) {
-            const int m = h_sigmol[A][r];
-            const int s0 = seg_start[B][m];
-            for (int j = s0; j < s0 + seg_len[B][m]; j++) {
+        for (int row = 0; row < nrows[cls]; row++) {
+            const int r = h_row_bra[cls][row], sg = h_row_seg[cls][row];
+            const int s0 = seg_start[B][sg];
+            for (int j = s0; j < s0 + seg_len[B][sg]; j++) {
                 if (!(h_sigv[A][r] * h_sigv[B][j] >= tol2) || (A == B && h_rank[B][j] > h_rank[A][r])) continue;
                 if (out && k < cap) {
                     const int bra = h_sig[A][r], ket = h_sig[B][j];
@@ -426,7 +477,7 @@ int GtfSystem::build_tiles()
         const int NA = 2 * ci->la + 1, NB = 2 * ci->lb + 1;
         const int NCD = (2 * ci->lc + 1) * (2 * ci->ld + 1);
         tile_lds[cls] = sizeof(double) * ((size_t)(NA + NB) * nbf + NA * NB +
-                                          (ci->nsplit > 1 ? (size_t)tile_kets(ci->nsplit) * NCD : 0));
+                                          (size_t)tile_kets(ci->nsplit) * NCD);
         ntiles[cls] = ntiles_lo[cls] = 0;
         nq_lo[cls] = 0;
         nprimq_lo[cls] = 0.0;
@@ -436,13 +487,39 @@ int GtfSystem::build_tiles()
         if (nq[cls] == 0 || !use_tiles || tile_lds[cls] > max_lds) continue;
         const std::vector<double>& vs = h_sorted[B];
         const std::vector<double> kp = ket_prim_prefix(B);
-        const int tq = tile_kets(ci->nsplit);
-        std::vector<int4> hi, lo;
-        for (int r = 0; r < nsig[A]; r++) {
-            const long c = h_exact[cls][r];
+        const int ns = ci->nsplit;
+        // tiles of the kets [b, e) of one row, cut by work units and by
+        // primitive quartets; ich = bra primitives per unit
+        std::vector<std::pair<double, int4>> hi, lo;
+        auto cut = [&](std::vector<std::pair<double, int4>>& out, int r, int nb, long b, long e) {
+            long j = b;
+            while (j < e) {
+                const long start = j;
+                int nkmax = 0, ich = nb;
+                double cost = 0.0;
+                while (j < e) {
+                    const int nk = h_pnprim[h_sig[B][j]];
+                    const int nkm = std::max(nkmax, nk);
+                    int ic = std::max(1, std::min(nb, (GTF_UNIT_PQ + nkm - 1) / nkm));
+                    // keep one ket's units within a tile
+                    while (((nb + ic - 1) / ic) * ns > GTF_TILE_UNITS && ic < nb) ic++;
+                    const long units = (j - start + 1) * (long)((nb + ic - 1) / ic) * ns;
+                    const double c = cost + (double)nb * nk;
+                    if (j > start && (units > GTF_TILE_UNITS || c > GTF_TILE_PQ || j - start >= tile_kets(ns))) break;
+                    nkmax = nkm;
+                    ich = ic;
+                    cost = c;
+                    j++;
+                }
+                out.push_back({cost, make_int4(r, (int)start, (int)j, ich)});
+            }
+        };
+        for (int row = 0; row < nrows[cls]; row++) {
+            const long c = h_exact[cls][row];
             if (c == 0) continue;
-            const int m = h_sigmol[A][r];
-            const int s0 = seg_start[B][m], nket = seg_len[B][m];
+            const int r = h_row_bra[cls][row], sg = h_row_seg[cls][row];
+            const int s0 = seg_start[B][sg], nket = seg_len[B][sg];
+            const int nb = h_pnprim[h_sig[A][r]];
             long c64 = c;
             if (tau2 > 0.0) {
                 const double vk = h_sigv[A][r];
@@ -452,25 +529,32 @@ int GtfSystem::build_tiles()
             }
             const long pad = std::min<long>(nket, ((c + GTF_KET_BLOCK - 1) / GTF_KET_BLOCK) * GTF_KET_BLOCK);
             const long pad64 = std::min<long>(nket, ((c64 + GTF_KET_BLOCK - 1) / GTF_KET_BLOCK) * GTF_KET_BLOCK);
-            for (long j0 = 0; j0 < pad64; j0 += tq)
-                hi.push_back(make_int4(r, (int)(s0 + j0), (int)(s0 + std::min<long>(pad64, j0 + tq)), 0));
+            cut(hi, r, nb, s0, s0 + pad64);
             if (c > c64) {
                 const long b0 = (c64 / GTF_KET_BLOCK) * GTF_KET_BLOCK;
-                for (long j0 = b0; j0 < pad; j0 += tq)
-                    lo.push_back(make_int4(r, (int)(s0 + j0), (int)(s0 + std::min<long>(pad, j0 + tq)), 0));
+                cut(lo, r, nb, s0 + b0, s0 + pad);
                 nq_lo[cls] += c - c64;
-                nprimq_lo[cls] += (double)h_pnprim[h_sig[A][r]] * (kp[s0 + c] - kp[s0 + c64]);
+                nprimq_lo[cls] += (double)nb * (kp[s0 + c] - kp[s0 + c64]);
             }
         }
-        ntiles[cls] = (int)hi.size();
-        ntiles_lo[cls] = (int)lo.size();
-        if (!hi.empty()) {
-            d_tiles[cls] = dalloc<int4>(hi.size());
-            if (!d_tiles[cls] || upload(d_tiles[cls], hi, stream)) return -1;
+        // heaviest tiles first, so the light ones fill the tail
+        auto heavy_first = [](const std::pair<double, int4>& x, const std::pair<double, int4>& y) {
+            return x.first > y.first;
+        };
+        std::stable_sort(hi.begin(), hi.end(), heavy_first);
+        std::stable_sort(lo.begin(), lo.end(), heavy_first);
+        std::vector<int4> hv, lv;
+        for (auto& x : hi) hv.push_back(x.second);
+        for (auto& x : lo) lv.push_back(x.second);
+        ntiles[cls] = (int)hv.size();
+        ntiles_lo[cls] = (int)lv.size();
+        if (!hv.empty()) {
+            d_tiles[cls] = dalloc<int4>(hv.size());
+            if (!d_tiles[cls] || upload(d_tiles[cls], hv, stream)) return -1;
         }
-        if (!lo.empty()) {
-            d_tiles_lo[cls] = dalloc<int4>(lo.size());
-            if (!d_tiles_lo[cls] || upload(d_tiles_lo[cls], lo, stream)) return -1;
+        if (!lv.empty()) {
+            d_tiles_lo[cls] = dalloc<int4>(lv.size());
+            if (!d_tiles_lo[cls] || upload(d_tiles_lo[cls], lv, stream)) return -1;
         }
     }
     SYS_CHECK(hipStreamSynchronize(stream));
@@ -511,21 +595,22 @@ int GtfSystem::jk(const double* D, double* J, double* K)
         a.rank_ket = d_rank[B];
         a.tol2 = tol2;
         a.same = A == B;
-        a.nrows = nsig[A];
+        a.nrows = nrows[cls];
         a.bra_list = d_sig[A];
         a.ket_list = d_sig[B];
         a.counter = d_counters + cls;
         a.ntasks = (int)((a.nunits + 255) / 256);
         a.pv = pv;
         a.fstart = d_fstart;
-        a.boys = d_boys;
+        a.boys8 = d_boys8;
+    a.boys8f = d_boys8f;
         a.D = D;
         a.J = J;
         a.K = K;
         a.mbase = d_mbase;
         a.mld = d_mld;
-        a.bra_mol = d_sigmol[A];
-        a.ket_seg_start = d_seg_start[B];
+        a.row_bra = d_row_bra[cls];
+        a.row_kstart = d_row_kstart[cls];
         if (timing) SYS_CHECK(hipEventRecord(tev[2 * cls], cs));
         if (ntiles[cls] > 0 || ntiles_lo[cls] > 0) {
             a.nbf = nbf;
@@ -582,7 +667,8 @@ int GtfSystem::store(int cls, int nq_, const int2* d_qlist, double* d_out)
     a.qlist = d_qlist;
     a.pv = pair_view();
     a.fstart = d_fstart;
-    a.boys = d_boys;
+    a.boys8 = d_boys8;
+    a.boys8f = d_boys8f;
     a.out = d_out;
     return launch(ci->store, a, (int)((a.nunits + 255) / 256), stream);
 }

@@ -35,7 +35,9 @@ struct GtfSystem {
     int* d_mld = nullptr;     // per shell: leading dimension (n of its molecule)
     long* d_mol_off = nullptr;
     int* d_mol_n = nullptr;
-    double* d_boys = nullptr;
+    double* d_boys = nullptr;     // one-electron grid (BOYS_NGRID x BOYS_NCOL)
+    double* d_boys8 = nullptr;    // ERI Taylor tables, FP64
+    float* d_boys8f = nullptr;    // and FP32
     // ---- shell pairs of every molecule, canonical orientation L(s1) >= L(s2)
     int npairs = 0;
     std::vector<int> h_s1, h_s2, h_type, h_pnprim;
@@ -54,23 +56,37 @@ struct GtfSystem {
     double vmax = 0.0;  // of molecule 0
     double tol = 0.0, tol2 = 0.0;
     bool screened = false;
-    // significant pairs per type: one segment per molecule, each segment
-    // value-sorted (descending) and then re-sorted by primitive count within
-    // blocks of GTF_KET_BLOCK; rank = position in the segment's value order
+    // Significant pairs per type, in segments: segment m * NBUCKET + b holds
+    // the pairs of molecule m whose primitive-pair count falls in bucket b
+    // (so a wavefront's kets have similar loop lengths), value-sorted
+    // (descending) and then re-sorted by primitive count within blocks of
+    // GTF_KET_BLOCK.  rank = position in the molecule's value order over all
+    // buckets; h_sorted / h_sortrank / h_sortnprim list each segment's values,
+    // ranks and primitive counts in value order.
+    static constexpr int NBUCKET = 5;
+    static int bucket(int nprim) { return nprim <= 1 ? 0 : nprim <= 4 ? 1 : nprim <= 12 ? 2 : nprim <= 32 ? 3 : 4; }
+    int nseg = 0;
     int nsig[GTF_NPAIRTYPES] = {0};
     std::vector<int> h_sig[GTF_NPAIRTYPES], h_rank[GTF_NPAIRTYPES], h_sigmol[GTF_NPAIRTYPES];
     std::vector<double> h_sigv[GTF_NPAIRTYPES], h_sorted[GTF_NPAIRTYPES];
+    std::vector<int> h_sortrank[GTF_NPAIRTYPES], h_sortnprim[GTF_NPAIRTYPES];
     std::vector<int> seg_start[GTF_NPAIRTYPES], seg_len[GTF_NPAIRTYPES];
     int* d_sig[GTF_NPAIRTYPES] = {nullptr};
     double* d_sigv[GTF_NPAIRTYPES] = {nullptr};
     int* d_rank[GTF_NPAIRTYPES] = {nullptr};
-    int* d_sigmol[GTF_NPAIRTYPES] = {nullptr};
     double* d_sigv_sorted[GTF_NPAIRTYPES] = {nullptr};
+    int* d_sortrank[GTF_NPAIRTYPES] = {nullptr};
     int* d_seg_start[GTF_NPAIRTYPES] = {nullptr};
     int* d_seg_len[GTF_NPAIRTYPES] = {nullptr};
-    // ---- quartet rows and tiles per class
+    // ---- quartet rows and tiles per class: a row is (bra pair, ket segment
+    // of the bra's molecule)
     long nq[GTF_NCLASSES] = {0};      // screened unique quartets
     long nslots[GTF_NCLASSES] = {0};  // row slots incl. the block padding
+    int nrows[GTF_NCLASSES] = {0};
+    std::vector<int> h_row_bra[GTF_NCLASSES], h_row_seg[GTF_NCLASSES];
+    int* d_row_bra[GTF_NCLASSES] = {nullptr};
+    int* d_row_seg[GTF_NCLASSES] = {nullptr};
+    int* d_row_kstart[GTF_NCLASSES] = {nullptr};
     long* d_rowoff[GTF_NCLASSES] = {nullptr};
     std::vector<long> h_exact[GTF_NCLASSES];
     double nprimq[GTF_NCLASSES] = {0};  // sum over quartets of nprim(bra) nprim(ket)

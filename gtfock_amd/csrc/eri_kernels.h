@@ -31,7 +31,8 @@ struct EriArgs {
     int ntasks;
     PairView pv;
     const int* fstart;
-    const double* boys;
+    const double* boys8;  // compact Taylor tables, BOYS8_NM slabs of BOYS8_SLAB
+    const float* boys8f;
     // EPI_JK
     const double* D;
     double* J;
@@ -56,9 +57,10 @@ struct EriArgs {
     // packed per-molecule matrices: block offset and leading dimension per shell
     const long* mbase;
     const int* mld;
-    // global-atomic path: molecule of each bra entry, ket segment start per molecule
-    const int* bra_mol;
-    const int* ket_seg_start;
+    // global-atomic path: bra pair (index into bra_list) and absolute first
+    // ket of each row
+    const int* row_bra;
+    const int* row_kstart;
 };
 
 // the Schwarz test of pfock/fock_task.c:192 and, for a diagonal class, the
@@ -84,13 +86,8 @@ __device__ __forceinline__ void digest(const EriArgs& a, int bra, int ket, int c
                                        const double (&acc)[NC][EriShape<LA, LB, LC, LD>::NCD], double* jmn,
                                        double* strip, int nbf, double* jpq);
 
-// the Boys-grid columns L..L+6 of one class staged in LDS (40 KB)
-#ifndef GTF_BOYS_LDS
-#define GTF_BOYS_LDS 0
-#endif
-
-template <int LA, int LB, int LC, int LD, int NC, int EPI, int BSTRIDE>
-__device__ __forceinline__ void eri_unit(const EriArgs& a, long u, const double* __restrict__ boys)
+template <int LA, int LB, int LC, int LD, int NC, int EPI>
+__device__ __forceinline__ void eri_unit(const EriArgs& a, long u)
 {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr int NSPLIT = S::NAB / NC;
@@ -103,14 +100,15 @@ __device__ __forceinline__ void eri_unit(const EriArgs& a, long u, const double*
         bra = bk.x;
         ket = bk.y;
     } else {
-        const int r = find_row(a.row_off, a.nrows, q);
-        const int j = a.ket_seg_start[a.bra_mol[r]] + (int)(q - a.row_off[r]);
+        const int row = find_row(a.row_off, a.nrows, q);
+        const int r = a.row_bra[row];
+        const int j = a.row_kstart[row] + (int)(q - a.row_off[row]);
         if (!quartet_valid(a, r, j)) return;
         bra = a.bra_list[r];
         ket = a.ket_list[j];
     }
     double acc[NC][NCD];
-    eri_block<LA, LB, LC, LD, NC, double, BSTRIDE>(a.pv, bra, ket, c0, boys, acc);
+    eri_block<LA, LB, LC, LD, NC, double>(a.pv, bra, ket, c0, a.boys8 + (LA + LB + LC + LD) * BOYS8_SLAB, acc);
 
     if constexpr (EPI == EPI_STORE) {
         double* o = a.out + q * (long)(S::NAB * NCD) + (long)c0 * NCD;
@@ -189,8 +187,11 @@ __device__ __forceinline__ void digest(const EriArgs& a, int bra, int ket, int c
                 constexpr int c = decltype(c_)::value;
                 s += dmn[c] * acc[c][cd];
             });
-            if (jpq) atomicAdd(&jpq[cd], cJ2 * s);
-            else GTF_ATOMIC_ADD(&J[(long)(oP + cd / ND) * ld + oQ + cd % ND], cJ2 * s);
+            if (jpq) {
+                atomicAdd(&jpq[cd], cJ2 * s);
+            } else {
+                GTF_ATOMIC_ADD(&J[(long)(oP + cd / ND) * ld + oQ + cd % ND], cJ2 * s);
+            }
         });
     }
     // K_MP -= cMP sum_q D_nq I,  K_NP -= cNP sum_q D_mq I
@@ -244,20 +245,6 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
 {
     if (a.counter) {
         __shared__ int s_task;
-#if GTF_BOYS_LDS
-        constexpr int L = LA + LB + LC + LD;
-        __shared__ double s_boys[BOYS_NGRID * BOYS_NTAYLOR];
-        for (int k = threadIdx.x; k < BOYS_NGRID * BOYS_NTAYLOR; k += 256) {
-            const int row = k / BOYS_NTAYLOR, col = k - row * BOYS_NTAYLOR;
-            s_boys[k] = a.boys[row * BOYS_NCOL + L + col];
-        }
-        __syncthreads();
-        const double* boys = s_boys;
-        constexpr int BS = BOYS_NTAYLOR;
-#else
-        const double* boys = a.boys;
-        constexpr int BS = BOYS_NCOL;
-#endif
         for (;;) {
             if (threadIdx.x == 0) s_task = atomicAdd(a.counter, 1);
             __syncthreads();
@@ -265,23 +252,32 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
             __syncthreads();
             if (task >= a.ntasks) break;
             const long u = (long)task * 256 + threadIdx.x;
-            if (u < a.nunits) eri_unit<LA, LB, LC, LD, NC, EPI, BS>(a, u, boys);
+            if (u < a.nunits) eri_unit<LA, LB, LC, LD, NC, EPI>(a, u);
         }
     } else {
         const long u = (long)blockIdx.x * 256 + threadIdx.x;
-        if (u < a.nunits) eri_unit<LA, LB, LC, LD, NC, EPI, BOYS_NCOL>(a, u, a.boys);
+        if (u < a.nunits) eri_unit<LA, LB, LC, LD, NC, EPI>(a, u);
     }
 }
 
 // Fused ERI + J/K digestion over bra-row tiles.  A tile is up to
 // GTF_TILE_UNITS work units of one bra row; the workgroup accumulates J_MN and
 // the exchange blocks of the bra shells in LDS and flushes them with one HBM
-// atomic per touched element when the tile is done.  When several threads
-// share a quartet (NSPLIT > 1) the J_PQ block of every ket of the tile is
-// also summed in LDS first.  Dynamic LDS: (NA + NB) * nbf + NA * NB doubles,
-// nbf = the largest molecule's size, plus tile_kets * NCD doubles when
-// NSPLIT > 1.
+// atomic per touched element when the tile is done.  The J_PQ block of
+// every ket of the tile is also kept in LDS and flushed at the end, so no
+// HBM atomic (whose completion the next loads would wait for) sits in the
+// quartet loop.  Dynamic LDS: (NA + NB) * nbf + NA * NB + tile_kets * NCD
+// doubles, nbf = the largest molecule's size.
+//
+// A tile is (bra row, first ket, end ket, bra-primitive chunk ich): each of
+// its quartets is split into ceil(nprim(bra) / ich) * NSPLIT work units, so
+// that a unit holds about GTF_UNIT_PQ primitive quartets however contracted
+// the pairs are, and a tile holds at most GTF_TILE_UNITS units and about
+// GTF_TILE_PQ primitive quartets.  Digestion is linear in the integrals, so
+// every unit digests its partial sum on its own.
 #define GTF_TILE_UNITS 1024
+#define GTF_UNIT_PQ 64
+#define GTF_TILE_PQ 65536
 
 // kets per tile of a class with nsplit threads per quartet
 __host__ __device__ constexpr int tile_kets(int nsplit) { return GTF_TILE_UNITS / nsplit > 0 ? GTF_TILE_UNITS / nsplit : 1; }
@@ -293,7 +289,8 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
     constexpr int NSPLIT = S::NAB / NC;
     constexpr int NA = S::NA, NB = S::NB, NCD = S::NCD;
     constexpr bool LOW = std::is_same<Real, float>::value;
-    constexpr bool JPQ_LDS = NSPLIT > 1;
+    constexpr bool JPQ_LDS = true;
+    constexpr int L = LA + LB + LC + LD;
     constexpr int ND = S::ND;
     extern __shared__ double smem[];
     __shared__ int s_task;
@@ -304,6 +301,8 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
         if (task >= a.ntiles) break;
         const int4 t = a.tiles[task];
         const int bra = a.bra_list[t.x];
+        const int ich = t.w;
+        const int nic = (a.pv.nprim[bra] + ich - 1) / ich;
         // strip rows are as wide as the bra's molecule (at most a.nbf)
         const int nbf = a.mld[a.pv.s1[bra]];
         const int nstrip = (NA + NB) * nbf;
@@ -313,10 +312,14 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
         const int njpq = JPQ_LDS ? (t.z - t.y) * NCD : 0;
         for (int k = threadIdx.x; k < nstrip + NA * NB + njpq; k += 256) smem[k] = 0.0;
         __syncthreads();
-        const int nunits = (t.z - t.y) * NSPLIT;
+        const int per_ket = nic * NSPLIT;
+        const int nunits = (t.z - t.y) * per_ket;
         for (int u = threadIdx.x; u < nunits; u += 256) {
-            const int q = u / NSPLIT;
-            const int c0 = (u - q * NSPLIT) * NC;
+            const int q = u / per_ket;
+            const int rem = u - q * per_ket;
+            const int ic = rem / NSPLIT;
+            const int c0 = (rem - ic * NSPLIT) * NC;
+            const int i0 = ic * ich, i1 = i0 + ich;
             const int j = t.y + q;
             if (!quartet_valid(a, t.x, j)) continue;
             // precision split on the Schwarz bound sqrt(v_bra v_ket) vs tau
@@ -325,14 +328,16 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
             double acc[NC][NCD];
             if constexpr (LOW) {
                 float accf[NC][NCD];
-                eri_block<LA, LB, LC, LD, NC, float, BOYS_NCOL>(a.pv, bra, ket, c0, a.boys, accf);
+                eri_block<LA, LB, LC, LD, NC, float>(a.pv, bra, ket, c0, a.boys8f + L * BOYS8_SLAB, accf, i0,
+                                                     min(i1, a.pv.nprim[bra]));
                 static_for<0, NC>([&](auto c_) {
                     static_for<0, NCD>([&](auto d_) {
                         acc[decltype(c_)::value][decltype(d_)::value] = (double)accf[decltype(c_)::value][decltype(d_)::value];
                     });
                 });
             } else {
-                eri_block<LA, LB, LC, LD, NC, double, BOYS_NCOL>(a.pv, bra, ket, c0, a.boys, acc);
+                eri_block<LA, LB, LC, LD, NC, double>(a.pv, bra, ket, c0, a.boys8 + L * BOYS8_SLAB, acc, i0,
+                                                      min(i1, a.pv.nprim[bra]));
             }
             digest<LA, LB, LC, LD, NC, true>(a, bra, ket, c0, acc, jmn, strip, nbf,
                                              JPQ_LDS ? jpq + q * NCD : nullptr);

@@ -86,14 +86,16 @@ __host__ __device__ constexpr int default_nc()
 }
 
 // Contracted integrals (bra components c0..c0+NC-1, all ket components) of
-// the quartet (bra pair | ket pair).  acc[c][cd], cd = c_sph * nsph(LD) + d_sph.
+// the quartet (bra pair | ket pair), summed over the bra primitive pairs
+// [i0, i1) (i1 < 0: all).  acc[c][cd], cd = c_sph * nsph(LD) + d_sph.
 // The ket coefficients are read in the compact nonzero-only layout and the
 // structurally zero terms are skipped at compile time; when a thread owns the
 // whole bra pair (NC == NAB, c0 == 0) the bra side is treated the same way.
-template <int LA, int LB, int LC, int LD, int NC, typename Real, int BSTRIDE = BOYS_NCOL>
+template <int LA, int LB, int LC, int LD, int NC, typename Real>
 __device__ __forceinline__ void eri_block(const PairView& pv, int bra, int ket, int c0,
-                                          const double* __restrict__ boys_tab,
-                                          Real (&acc)[NC][EriShape<LA, LB, LC, LD>::NCD])
+                                          const Real* __restrict__ boys_slab,
+                                          Real (&acc)[NC][EriShape<LA, LB, LC, LD>::NCD], int i0 = 0,
+                                          int i1 = -1)
 {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr int L = S::L, HAB = S::HAB, HCD = S::HCD, HL = S::HL, NCD = S::NCD;
@@ -109,7 +111,8 @@ __device__ __forceinline__ void eri_block(const PairView& pv, int bra, int ket, 
     const double4* __restrict__ pk = pv.prim + pv.poff[ket];
     const double* __restrict__ Eb = BRA_FULL ? pv.Ez + pv.zoff[bra] : pv.E + pv.eoff[bra] + (long)c0 * HAB;
     const double* __restrict__ Ek0 = pv.Ez + pv.zoff[ket];
-    for (int i = 0; i < nb; ++i) {
+    const int ie = i1 < 0 ? nb : i1;
+    for (int i = i0; i < ie; ++i) {
         const double4 B = pb[i];
         Real eb[NC][HAB];
         static_for<0, NC>([&](auto c_) {
@@ -131,7 +134,7 @@ __device__ __forceinline__ void eri_block(const PairView& pv, int bra, int ket, 
             const Real alpha = p * q * (r * r);
             const Real T = alpha * (X * X + Y * Y + Z * Z);
             Real F[L + 1];
-            boys_eval<L, Real, BSTRIDE>(T, F, boys_tab);
+            boys_eval8<L, Real>(T, F, boys_slab);
             const Real pref = Real(GTF_TWO_PI_2_5) * r;
             Real R[HL];
             hermite_R<L, Real>(R, F, pref, Real(-2) * alpha, X, Y, Z);

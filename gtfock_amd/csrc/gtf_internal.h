@@ -28,6 +28,14 @@
 #define BOYS_NCOL 16
 #define BOYS_TMAX 36.0
 #define BOYS_NTAYLOR 7
+// compact Taylor table of the ERI kernels: orders M = 0 .. BOYS8_NM - 1,
+// T0 = i * BOYS8_DELTA for i < BOYS8_NGRID, BOYS8_NTERM coefficients per row
+#define BOYS8_DELTA 0.1
+#define BOYS8_INV_DELTA 10.0
+#define BOYS8_NGRID 361
+#define BOYS8_NTERM 8
+#define BOYS8_NM 9
+#define BOYS8_SLAB (BOYS8_NGRID * BOYS8_NTERM)
 
 #define GTF_CHECK(call)                                                                    \
     do {                                                                                   \

@@ -5,9 +5,9 @@
 int gtf_build_pairs(hipStream_t st, const BasisView& b, int npairs, const int* s1, const int* s2,
                     const int* poff, const long* eoff, double4* prim, double* E, const long* zoff, double* Ez,
                     int* nprim_out, double prim_eps);
-int gtf_count_rows(hipStream_t st, int nbra, const double* vbra, const int* rank_bra, const int* bra_mol,
-                   const double* vket_sorted, const int* seg_start, const int* seg_len, double tol2, int same,
-                   long* exact, long* counts, long* row_off);
+int gtf_count_rows(hipStream_t st, int nrows, const int* row_bra, const int* row_seg, const double* vbra,
+                   const int* rank_bra, const double* vket_sorted, const int* rank_sorted, const int* seg_start,
+                   const int* seg_len, double tol2, int same, long* exact, long* counts, long* row_off);
 int gtf_symmetrize_packed(hipStream_t st, double* A, int nmol, const long* off, const int* n, int maxn);
 int gtf_fill_qlist_diag(hipStream_t st, int n, const int* ids, int2* q);
 

@@ -93,22 +93,25 @@ __global__ void build_pairs_kernel(BasisView b, int npairs, const int* __restric
     nprim_out[pid] = k;
 }
 
-// exact[k] = number of ket pairs j of the bra's molecule (value order) with
-// v_bra[k] * v_ket[j] >= tol2 (the test of pfock/fock_task.c:192); for a
-// diagonal class only rank(j) <= rank(k), so each unordered pair of pairs
-// appears once.  counts[k] rounds exact[k] up to whole ket blocks within the
-// molecule's segment (the kernels re-apply the exact test to the padding).
-__global__ void count_rows_kernel(int nbra, const double* __restrict__ vbra, const int* __restrict__ rank_bra,
-                                  const int* __restrict__ bra_mol, const double* __restrict__ vket_sorted,
+// Row k = (bra pair r, ket segment s).  exact[k] = number of kets j of the
+// segment (value order) with v_r * v_j >= tol2 (the test of
+// pfock/fock_task.c:192); for a diagonal class also rank(j) <= rank(r), so
+// each unordered pair of pairs appears once (ranks rise along a segment's
+// value order, so both conditions select prefixes).  counts[k] rounds
+// exact[k] up to whole ket blocks within the segment (the kernels re-apply
+// the exact test to the padding).
+__global__ void count_rows_kernel(int nrows, const int* __restrict__ row_bra, const int* __restrict__ row_seg,
+                                  const double* __restrict__ vbra, const int* __restrict__ rank_bra,
+                                  const double* __restrict__ vket_sorted, const int* __restrict__ rank_sorted,
                                   const int* __restrict__ seg_start, const int* __restrict__ seg_len, double tol2,
                                   int same, long* __restrict__ exact, long* __restrict__ counts)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nbra) return;
-    const double vk = vbra[k];
-    const int m = bra_mol[k];
-    const double* vs = vket_sorted + seg_start[m];
-    const int nket = seg_len[m];
+    if (k >= nrows) return;
+    const int r = row_bra[k], s = row_seg[k];
+    const double vk = vbra[r];
+    const double* vs = vket_sorted + seg_start[s];
+    const int nket = seg_len[s];
     int lo = 0, hi = nket;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -116,7 +119,17 @@ __global__ void count_rows_kernel(int nbra, const double* __restrict__ vbra, con
         else hi = mid;
     }
     long cnt = lo;
-    if (same && cnt > rank_bra[k] + 1) cnt = rank_bra[k] + 1;
+    if (same) {
+        const int* rs = rank_sorted + seg_start[s];
+        const int rk = rank_bra[r];
+        int a = 0, b = nket;  // number of entries with rank <= rk
+        while (a < b) {
+            const int mid = (a + b) >> 1;
+            if (rs[mid] <= rk) a = mid + 1;
+            else b = mid;
+        }
+        if (cnt > a) cnt = a;
+    }
     exact[k] = cnt;
     long pad = cnt == 0 ? 0 : ((cnt + GTF_KET_BLOCK - 1) / GTF_KET_BLOCK) * GTF_KET_BLOCK;
     counts[k] = pad < nket ? pad : nket;
@@ -168,15 +181,15 @@ int gtf_build_pairs(hipStream_t st, const BasisView& b, int npairs, const int* s
     return 0;
 }
 
-int gtf_count_rows(hipStream_t st, int nbra, const double* vbra, const int* rank_bra, const int* bra_mol,
-                   const double* vket_sorted, const int* seg_start, const int* seg_len, double tol2, int same,
-                   long* exact, long* counts, long* row_off)
+int gtf_count_rows(hipStream_t st, int nrows, const int* row_bra, const int* row_seg, const double* vbra,
+                   const int* rank_bra, const double* vket_sorted, const int* rank_sorted, const int* seg_start,
+                   const int* seg_len, double tol2, int same, long* exact, long* counts, long* row_off)
 {
-    if (nbra <= 0) return 0;
-    count_rows_kernel<<<(nbra + 255) / 256, 256, 0, st>>>(nbra, vbra, rank_bra, bra_mol, vket_sorted, seg_start,
-                                                          seg_len, tol2, same, exact, counts);
+    if (nrows <= 0) return 0;
+    count_rows_kernel<<<(nrows + 255) / 256, 256, 0, st>>>(nrows, row_bra, row_seg, vbra, rank_bra, vket_sorted,
+                                                           rank_sorted, seg_start, seg_len, tol2, same, exact, counts);
     GTF_CHECK(hipGetLastError());
-    scan_kernel<<<1, 1024, 0, st>>>(counts, row_off, nbra);
+    scan_kernel<<<1, 1024, 0, st>>>(counts, row_off, nrows);
     GTF_CHECK(hipGetLastError());
     return 0;
 }
