@@ -28,6 +28,11 @@ FP64_PEAK = 78.6e12
 HBM_PEAK = 8.0e12
 
 
+def log(msg):
+    """Progress of the legs, on stderr (the JSON line alone goes to stdout)."""
+    print("[bench %s] %s" % (time.strftime("%H:%M:%S"), msg), file=sys.stderr, flush=True)
+
+
 def device_sync():
     try:
         import torch
@@ -144,27 +149,106 @@ def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters):
            "mixed_vs_unbatched_fp64": t_single / t_b32, "batched_fp64_vs_unbatched": t_single / t_b64,
            "fp32_quartet_fraction": st["nq_fp32"] / max(1.0, st["nquartets"]),
            "max_abs_dF": dF, "max_first_order_dE_Eh": dE1}
+    log("batched: builds timed; SCF energy check")
     if scf_iters > 0:
+        # a molecule counts when its FP64 SCF converged (|dE| < 1e-8 Eh between
+        # builds) within scf_iters builds; the reference algorithm (zero initial
+        # density, trace-correcting purification, DIIS) does not converge for
+        # every geometry, and those are reported, not compared
         b64 = g.Batch(bases)
+        t0 = time.perf_counter()
         e64, it64 = b64.scf(max_iters=scf_iters, ediff=1e-8)
+        t_scf64 = time.perf_counter() - t0
         b64.close()
         b32 = g.Batch(bases, tau=tau)
         t0 = time.perf_counter()
         e32, it32 = b32.scf(max_iters=scf_iters, ediff=1e-8)
         t_scf = time.perf_counter() - t0
         b32.close()
-        out.update({"scf_max_abs_dE_Eh": float(np.abs(e32 - e64).max()),
-                    "scf_builds_mean": float(np.mean(it32)), "scf_mixed_s": t_scf})
+        conv = (it64 > 0) & (it64 < scf_iters)
+        de = np.abs(e32 - e64)[conv]
+        out.update({"scf_converged": int(conv.sum()), "scf_not_converged": int((~conv).sum()),
+                    "scf_max_abs_dE_Eh": float(de.max()) if de.size else None,
+                    "scf_builds_mean": float(np.mean(it64[conv])) if conv.any() else None,
+                    "scf_fp64_s": t_scf64, "scf_mixed_s": t_scf})
     return out
 
 
-def load_traffic(workload, cls_name):
+def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max_iters, ref):
+    """SCF requests with seeded Poisson arrivals (rate jobs/s) submitted in
+    real time to the request server (mixed precision, HBM-budget admission,
+    several cohorts on the device at once).  Time to converge = done time -
+    submit time per job; throughput = jobs / (last done - first submit).
+    With ref, the same molecules are run afterwards as one FP64 batch and
+    the largest total-energy difference over jobs converged in both is
+    reported."""
+    from gtfock_amd import geometry
+    tmp = tempfile.mkdtemp()
+    mols = geometry.serve_molecules(njobs, seed, max_rings)
+    bases = []
+    for k, (name, atoms) in enumerate(mols):
+        p = os.path.join(tmp, "j%03d.xyz" % k)
+        with open(p, "w") as fh:
+            fh.write(geometry.to_xyz(atoms))
+        bases.append(g.BasisSet(gbs, p))
+    rng = np.random.default_rng(seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=njobs))
+    srv = g.Server(hbm_budget_gb=budget_gb, max_cohorts=cohorts, max_jobs_per_cohort=64, tau=tau,
+                   max_iters=max_iters, ediff=1e-8)
+    t0 = time.time()
+    ids = []
+    for b, at in zip(bases, arrivals):
+        dt = t0 + at - time.time()
+        if dt > 0:
+            time.sleep(dt)
+        ids.append(srv.submit(b))
+    log("serve: %d jobs submitted" % njobs)
+    done = set()
+    while len(done) < njobs:
+        time.sleep(0.05)
+        for k, i in enumerate(ids):
+            if k in done:
+                continue
+            r = srv.poll(i)
+            if r["status"] not in (0, 1):
+                done.add(k)
+                log("serve: job %d (%s, nbf %d) status %d after %.2f s, %d builds" % (
+                    k, mols[k][0], bases[k].nbf, r["status"], r["t_done"] - r["t_submit"], r["iters"]))
+    srv.wait()
+    res = [srv.poll(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    ok = [r for r in res if r["status"] == 2]
+    lat = np.array([r["t_done"] - r["t_submit"] for r in ok])
+    span = max(r["t_done"] for r in ok) - min(r["t_submit"] for r in ok)
+    nbf = [b.nbf for b in bases]
+    out = {"jobs": njobs, "completed": len(ok), "rate_jobs_per_s": rate, "seed": seed,
+           "flake_rings_max": max_rings, "nbf_max": max(nbf), "nbf_mean": float(np.mean(nbf)),
+           "tau": tau, "hbm_budget_gb": budget_gb, "max_cohorts": cohorts, "cohorts": int(st["cohorts"]),
+           "peak_running_cohorts": int(st["peak_running"]), "peak_gb": st["peak_bytes"] / 2 ** 30,
+           "throughput_jobs_per_s": len(ok) / span, "p50_s": float(np.percentile(lat, 50)),
+           "p99_s": float(np.percentile(lat, 99)), "mean_builds": float(np.mean([r["iters"] for r in ok])),
+           "not_converged": int(sum(1 for r in ok if r["iters"] >= max_iters or r["iters"] < 0))}
+    if ref:
+        log("serve: FP64 reference batch of the %d molecules" % njobs)
+        bt = g.Batch(bases)
+        e64, it64 = bt.scf(max_iters=max_iters, ediff=1e-8)
+        bt.close()
+        errs = [abs(r["energy"] - e64[k]) for k, r in enumerate(res)
+                if r["status"] == 2 and 0 < r["iters"] < max_iters and 0 < it64[k] < max_iters]
+        out["max_abs_dE_vs_fp64_Eh"] = float(max(errs)) if errs else None
+        out["dE_compared_jobs"] = len(errs)
+    return out
+
+
+def load_traffic(workload, kernel):
+    """HBM bytes per launch of a kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (profiles/eri_traffic_<workload>.json)."""
     path = os.path.join(ROOT, "profiles", "eri_traffic_%s.json" % workload)
     if not os.path.exists(path):
         return None
     try:
-        t = json.load(open(path))
-        return t.get(cls_name)
+        return json.load(open(path))["kernels"].get(kernel)
     except Exception:
         return None
 
@@ -181,7 +265,13 @@ def main():
     ap.add_argument("--classes", action="store_true", help="print the per-class breakdown to stderr")
     ap.add_argument("--batch", type=int, default=32, help="molecules of the batched leg (0: skip)")
     ap.add_argument("--batch-seed", type=int, default=20261202)
-    ap.add_argument("--batch-scf", type=int, default=0, help="SCF iterations of the batched energy check")
+    ap.add_argument("--batch-scf", type=int, default=30, help="SCF iterations of the batched energy check (0: skip)")
+    ap.add_argument("--serve", type=int, default=16, help="jobs of the serving leg (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, jobs/s")
+    ap.add_argument("--serve-rings", type=int, default=3, help="largest graphene flake (rings) in the request stream")
+    ap.add_argument("--serve-budget", type=float, default=64.0, help="HBM budget of the server, GiB")
+    ap.add_argument("--serve-cohorts", type=int, default=4)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -198,6 +288,7 @@ def main():
         fh.write(geometry.to_xyz(molecule(args.workload)))
 
     basis = g.BasisSet(gbs, xyz)
+    log("%s: PFock_create" % args.workload)
     t0 = time.perf_counter()
     pf = g.PFock(basis, tolscr=args.tol)
     t_create = time.perf_counter() - t0
@@ -207,6 +298,7 @@ def main():
     for _ in range(args.warmup):
         pf.compute_fock()
     pf.set_timing(True)
+    log("%s: %d timed FP64 builds" % (args.workload, args.steps))
     device_sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -238,39 +330,50 @@ def main():
                 "the ERI kernels run on the FP64 VALU)",
                 "kernel": pf.class_kernel(dom), "class": names[dom],
                 "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK, "traffic": load_traffic(args.workload, names[dom]),
+                "frac": achieved / FP64_PEAK, "traffic": load_traffic(args.workload, pf.class_kernel(dom)),
                 "algorithmic_flops_per_launch": dom_fl, "avg_launch_ms": dom_s * 1e3,
                 "build_tflops": total_fl / (ms_step * 1e-3) / 1e12}
 
-    # mixed precision on the same molecule and density: FP32 ERIs for quartets
-    # whose Schwarz bound is below tau, FP64 J/K accumulation
-    F64 = pf.get_mat(g.MAT_F)
-    pf.set_timing(False)
-    pf.set_precision(args.tau)
-    for _ in range(args.warmup):
-        pf.compute_fock()
-    device_sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        pf.compute_fock()
-    device_sync()
-    t1 = time.perf_counter()
-    F32 = pf.get_mat(g.MAT_F)
-    nq_lo, _ = pf.mixed_counts()
-    mixed = {"value": args.steps / (t1 - t0), "unit": "Fock builds/s", "tau": args.tau,
-             "ms_per_step": (t1 - t0) * 1e3 / args.steps,
-             "speedup_vs_fp64": (args.steps / (t1 - t0)) / value,
-             "fp32_quartet_fraction": float(nq_lo.sum()) / max(1.0, float(nq.sum())),
-             "max_abs_dF": float(np.abs(F32 - F64).max()),
-             "dE_Eh": float(abs(np.sum(D * (F32 - F64))))}
-    pf.set_precision(0.0)
+    mixed = None
+    if not args.no_mixed:
+        # mixed precision on the same molecule and density: FP32 ERIs for quartets
+        # whose Schwarz bound is below tau, FP64 J/K accumulation
+        F64 = pf.get_mat(g.MAT_F)
+        pf.set_timing(False)
+        pf.set_precision(args.tau)
+        log("%s: %d timed mixed-precision builds" % (args.workload, args.steps))
+        for _ in range(args.warmup):
+            pf.compute_fock()
+        device_sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            pf.compute_fock()
+        device_sync()
+        t1 = time.perf_counter()
+        F32 = pf.get_mat(g.MAT_F)
+        nq_lo, _ = pf.mixed_counts()
+        mixed = {"value": args.steps / (t1 - t0), "unit": "Fock builds/s", "tau": args.tau,
+                 "ms_per_step": (t1 - t0) * 1e3 / args.steps,
+                 "speedup_vs_fp64": (args.steps / (t1 - t0)) / value,
+                 "fp32_quartet_fraction": float(nq_lo.sum()) / max(1.0, float(nq.sum())),
+                 "max_abs_dF": float(np.abs(F32 - F64).max()),
+                 "dE_Eh": float(abs(np.sum(D * (F32 - F64))))}
+        pf.set_precision(0.0)
+
+    served = None
+    if args.serve > 0:
+        log("serve leg: %d jobs at %.0f jobs/s" % (args.serve, args.serve_rate))
+        served = serve_leg(g, gbs, args.serve, args.serve_rate, 20261203, args.tau, args.serve_rings,
+                           args.serve_budget, args.serve_cohorts, 40, True)
 
     batched = None
     if args.batch > 0:
+        log("batched leg: %d molecules" % args.batch)
         batched = batch_leg(g, gbs, args.batch, args.batch_seed, args.tau, max(2, args.steps // 4), args.batch_scf)
 
     cpu = None
     if not args.no_cpu:
+        log("CPU baseline: one oracle Fock build")
         cpu = cpu_baseline(gbs, xyz, D, args.tol)
 
     st = pf.stats()
@@ -285,6 +388,7 @@ def main():
         "roofline": roofline,
         "mixed": mixed,
         "batched": batched,
+        "serve": served,
         "cpu_baseline": cpu,
     }
     pf.close()

@@ -576,6 +576,12 @@ int gtf_batch_class_work(gtf_batch_t b, long* nquartets, double* nprimquartets, 
 
 int gtf_batch_scf(gtf_batch_t b, int max_iters, double ediff, double* energy, int* iters, double* history)
 {
+    return gtf_batch_scf_cb(b, max_iters, ediff, energy, iters, history, nullptr, nullptr);
+}
+
+int gtf_batch_scf_cb(gtf_batch_t b, int max_iters, double ediff, double* energy, int* iters, double* history,
+                     gtf_scf_done_fn done, void* ctx)
+{
     if (!b || max_iters <= 0 || !energy) return -3;
     if (b->scf_setup()) return -1;
     const int nmol = b->nmol;
@@ -609,12 +615,14 @@ int gtf_batch_scf(gtf_batch_t b, int max_iters, double ediff, double* energy, in
                 active[m] = 0;
                 nit[m] = -(it + 1);
                 b->done_time[m] = bwtime();
+                if (done) done(ctx, m, energy[m], nit[m]);
                 continue;
             }
             if (it > 0 && fabs(e[m] - e0[m]) < ediff) {
                 active[m] = 0;
                 nit[m] = it + 1;
                 b->done_time[m] = bwtime();
+                if (done) done(ctx, m, energy[m], nit[m]);
             } else {
                 e0[m] = e[m];
                 nactive++;
@@ -634,7 +642,10 @@ int gtf_batch_scf(gtf_batch_t b, int max_iters, double ediff, double* energy, in
     }
     const double tend = bwtime();
     for (int m = 0; m < nmol; m++)
-        if (active[m]) b->done_time[m] = tend;
+        if (active[m]) {
+            b->done_time[m] = tend;
+            if (done) done(ctx, m, energy[m], nit[m]);
+        }
     if (iters)
         for (int m = 0; m < nmol; m++) iters[m] = nit[m];
     return 0;
@@ -645,6 +656,25 @@ int gtf_batch_done_times(gtf_batch_t b, double* t)
     if (!b || !t) return -3;
     for (int m = 0; m < b->nmol; m++) t[m] = b->done_time[m];
     return b->nmol;
+}
+
+// Device bytes of one molecule in a batch: shell-pair data (primitive pairs,
+// full and compact Hermite coefficients), screening lists, the packed and
+// padded SCF matrices and the DIIS history.
+double gtf_batch_job_bytes(BasisSet_t basis)
+{
+    if (!basis) return -1.0;
+    const BasisSet* b = basis;
+    double pairs = 0.0;
+    for (int s = 0; s < b->nshells; s++)
+        for (int t = 0; t <= s; t++) {
+            int la = b->L[s], lb = b->L[t];
+            if (lb > la) std::swap(la, lb);
+            const double np = (double)b->nprim[s] * b->nprim[t];
+            pairs += np * (32.0 + 8.0 * (nsph(la) * nsph(lb) * nherm(la + lb) + herm_nnz(la, lb))) + 96.0;
+        }
+    const double n = b->nbf, pl = ((b->nbf + 63) / 64) * 64;
+    return pairs * 1.25 + 8.0 * (9.0 * n * n + (8.0 + 2.0 * B_MAX_DIIS) * pl * pl) + (1 << 20);
 }
 
 int gtf_batch_get_density(gtf_batch_t b, int m, double* D)

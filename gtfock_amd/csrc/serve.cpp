@@ -1,17 +1,23 @@
 // serve.cpp -- an SCF request server over batches.
 //
 // Jobs (one molecule each) queue on submission.  A scheduler thread admits
-// them in arrival order into cohorts: a cohort takes every pending job that
-// fits the device-memory budget (per-job footprint estimated from the basis
-// before anything is allocated), up to a job cap, and runs as one batch on
-// its own thread and HIP streams.  Up to max_cohorts cohorts run at once, so
-// requests that arrive while others are on the GPU start as soon as a cohort
-// slot and memory are free, and their kernels overlap with the running
-// cohorts' on the device.  Inside a cohort a converged molecule stops
-// iterating at once (its SCF result is final) while the others continue;
-// the cohort's memory returns to the budget when its last molecule is done.
-// Every per-molecule quantity of a batch is computed independently of the
-// other molecules in it, so a job's result does not depend on its cohort.
+// them in arrival order into cohorts: the oldest queued job opens a cohort
+// and takes the queued jobs of its size class (basis functions in the same
+// power-of-two band, so a small molecule does not iterate in lockstep with a
+// large one), oldest first, up to a job cap and the device-memory budget
+// (per-job footprint estimated from the basis before anything is allocated).
+// A cohort runs as one batch on its own thread and HIP streams; up to
+// max_cohorts cohorts run at once, so requests that arrive while others are
+// on the GPU start as soon as a cohort slot and memory are free, and their
+// kernels overlap with the running cohorts' on the device.  A job's result is
+// published the moment its own SCF ends; the cohort's memory returns to the
+// budget when its last molecule is done.  Every per-molecule quantity of a
+// batch is computed independently of the other molecules in it, so a job's
+// result does not depend on its cohort.
+//
+// Only the library's public C API is used here (the batch entry points and
+// CInt_getNumFuncs), so the scheduler builds and can be tested on the host
+// against a stand-in batch.
 #include <sys/time.h>
 
 #include <algorithm>
@@ -23,8 +29,6 @@
 #include <vector>
 
 #include "../../include/gtfock_amd.h"
-#include "basis_internal.h"
-#include "gtf_internal.h"
 
 static double swtime()
 {
@@ -34,12 +38,13 @@ static double swtime()
 }
 
 struct SJob {
-    BasisSet* basis = nullptr;
+    BasisSet_t basis = nullptr;
     double t_submit = 0, t_start = 0, t_done = 0;
     double energy = 0;
     int iters = 0;
     int status = 0;  // 0 queued, 1 running, 2 done, < 0 failed
     int cohort = -1;
+    int size_class = 0;
     double mem_est = 0;
 };
 
@@ -61,21 +66,30 @@ struct GtfServer {
     void run_cohort(std::vector<long> ids, double mem, int cohort);
 };
 
-// Device bytes of one job: shell-pair data (primitive pairs, full and
-// compact Hermite coefficients), screening lists, and the packed and padded
-// SCF matrices with the DIIS history.
-static double job_bytes(const BasisSet* b)
+// power-of-two band of the basis size: 24..47 -> 0, 48..95 -> 1, ...
+static int size_class(int nbf)
 {
-    double pairs = 0.0;
-    for (int s = 0; s < b->nshells; s++)
-        for (int t = 0; t <= s; t++) {
-            int la = b->L[s], lb = b->L[t];
-            if (lb > la) std::swap(la, lb);
-            const double np = (double)b->nprim[s] * b->nprim[t];
-            pairs += np * (32.0 + 8.0 * (nsph(la) * nsph(lb) * nherm(la + lb) + herm_nnz(la, lb))) + 96.0;
-        }
-    const double n = b->nbf, pl = ((b->nbf + 63) / 64) * 64;
-    return pairs * 1.25 + 8.0 * (9.0 * n * n + 28.0 * pl * pl) + (1 << 20);
+    int c = 0;
+    for (int n = 48; nbf >= n && c < 16; n *= 2) c++;
+    return c;
+}
+
+struct CohortCtx {
+    GtfServer* s;
+    const std::vector<long>* ids;
+};
+
+// called on the cohort's thread when molecule m's SCF has ended
+static void publish(void* ctx, int m, double energy, int builds)
+{
+    CohortCtx* c = (CohortCtx*)ctx;
+    std::lock_guard<std::mutex> lk(c->s->mu);
+    SJob& j = c->s->jobs[(*c->ids)[m]];
+    j.energy = energy;
+    j.iters = builds;
+    j.t_done = swtime();
+    j.status = builds < 0 ? -5 : 2;
+    c->s->cv.notify_all();
 }
 
 void GtfServer::run_cohort(std::vector<long> ids, double mem, int cohort)
@@ -94,29 +108,24 @@ void GtfServer::run_cohort(std::vector<long> ids, double mem, int cohort)
     const int nmol = (int)ids.size();
     std::vector<double> e(nmol, 0.0);
     std::vector<int> it(nmol, 0);
-    std::vector<double> tdone(nmol, 0.0);
+    CohortCtx ctx{this, &ids};
     gtf_batch_t b = nullptr;
     int rc = gtf_batch_create(bases.data(), nmol, tolscr, &b);
     if (rc == 0 && tau > 0.0) rc = gtf_batch_set_precision(b, tau);
-    if (rc == 0) rc = gtf_batch_scf(b, max_iters, ediff, e.data(), it.data(), nullptr);
-    if (rc == 0) gtf_batch_done_times(b, tdone.data());
+    if (rc == 0) rc = gtf_batch_scf_cb(b, max_iters, ediff, e.data(), it.data(), nullptr, publish, &ctx);
     gtf_batch_destroy(b);
     std::lock_guard<std::mutex> lk(mu);
     const double t = swtime();
     for (int k = 0; k < nmol; k++) {
         SJob& j = jobs[ids[k]];
-        // a molecule's result is final when its own SCF converged
-        j.t_done = rc == 0 && tdone[k] > 0.0 ? tdone[k] : t;
-        if (rc == 0) {
-            j.energy = e[k];
-            j.iters = it[k];
-            j.status = 2;
-        } else {
+        if (j.status == 1) {
+            // a failed cohort (or one that ended without reporting the job)
             j.status = rc < 0 ? rc : -1;
+            j.t_done = t;
         }
     }
-    mem_in_use -= mem;
     running--;
+    mem_in_use = running == 0 ? 0.0 : mem_in_use - mem;  // no rounding residue once idle
     cv.notify_all();
 }
 
@@ -135,28 +144,41 @@ void GtfServer::schedule()
             cv.notify_all();
             break;
         }
+        // jobs larger than the whole budget can never run
+        for (auto itp = pending.begin(); itp != pending.end();) {
+            if (jobs[*itp].mem_est > budget) {
+                jobs[*itp].status = -2;
+                jobs[*itp].t_done = swtime();
+                itp = pending.erase(itp);
+                cv.notify_all();
+            } else {
+                ++itp;
+            }
+        }
+        if (pending.empty()) continue;
+        // the oldest job opens the cohort; same-class jobs join, oldest first
+        const int cls = jobs[pending.front()].size_class;
         std::vector<long> take;
         double mem = 0.0;
-        while (!pending.empty() && (int)take.size() < max_jobs) {
-            const long id = pending.front();
-            const double need = jobs[id].mem_est;
-            if (need > budget) {
-                // can never fit: fail it rather than block the queue
-                jobs[id].status = -2;
-                jobs[id].t_done = swtime();
-                pending.pop_front();
-                cv.notify_all();
+        for (auto itp = pending.begin(); itp != pending.end() && (int)take.size() < max_jobs;) {
+            const SJob& j = jobs[*itp];
+            if (j.size_class != cls) {
+                ++itp;
                 continue;
             }
-            if (mem_in_use + mem + need > budget) break;
-            take.push_back(id);
-            mem += need;
-            pending.pop_front();
+            if (mem_in_use + mem + j.mem_est > budget) {
+                if (take.empty()) break;  // the oldest job waits for memory
+                ++itp;
+                continue;
+            }
+            take.push_back(*itp);
+            mem += j.mem_est;
+            itp = pending.erase(itp);
         }
         if (take.empty()) {
-            // the head job waits for a running cohort to return its memory
+            // the oldest job waits for a running cohort to return memory
             // (mem_in_use > 0 here, so some cohort is running)
-            if (!pending.empty()) cv.wait(lk);
+            cv.wait(lk);
             continue;
         }
         mem_in_use += mem;
@@ -192,13 +214,17 @@ int gtf_server_create(double hbm_budget_bytes, int max_cohorts, int max_jobs_per
 
 long gtf_server_submit(gtf_server_t s, BasisSet_t basis)
 {
-    if (!s || !basis || basis->nshells == 0) return -3;
+    if (!s || !basis) return -3;
+    const int nbf = CInt_getNumFuncs(basis);
+    const double bytes = gtf_batch_job_bytes(basis);
+    if (nbf <= 0 || bytes <= 0) return -3;
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stopping) return -3;
     SJob j;
     j.basis = basis;
     j.t_submit = swtime();
-    j.mem_est = job_bytes(basis);
+    j.mem_est = bytes;
+    j.size_class = size_class(nbf);
     s->jobs.push_back(j);
     const long id = (long)s->jobs.size() - 1;
     s->pending.push_back(id);
@@ -228,10 +254,7 @@ int gtf_server_wait(gtf_server_t s)
 {
     if (!s) return -3;
     std::unique_lock<std::mutex> lk(s->mu);
-    s->cv.wait(lk, [&] {
-        if (!s->pending.empty() || s->running > 0) return false;
-        return true;
-    });
+    s->cv.wait(lk, [&] { return s->pending.empty() && s->running == 0; });
     return 0;
 }
 

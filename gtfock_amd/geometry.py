@@ -152,3 +152,25 @@ def batch_molecules(nmol, seed=20261202, max_alkane=20, max_cluster=16):
         k = int(rng.integers(1, max_cluster + 1))
         out.append(("(H2O)%d" % k, water_cluster(k, rng)))
     return out
+
+
+def serve_molecules(njobs, seed=20261203, max_rings=4, flake_fraction=0.3):
+    """A seeded SCF request stream: each job is, with probability
+    1 - flake_fraction, a molecule of the batched distribution (an all-trans
+    alkane with n uniform in 1..20 or a water cluster of k uniform in 1..16,
+    half each), else an H-capped graphene flake with 2..max_rings rings
+    (C24H12, C54H18, C96H24).  Returns [(name, atoms)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(njobs):
+        if rng.uniform() < flake_fraction:
+            rings = int(rng.integers(2, max_rings + 1))
+            atoms = graphene_flake(rings)
+            out.append((formula(atoms), atoms))
+        elif rng.uniform() < 0.5:
+            n = int(rng.integers(1, 21))
+            out.append(("C%dH%d" % (n, 2 * n + 2), alkane(n)))
+        else:
+            k = int(rng.integers(1, 17))
+            out.append(("(H2O)%d" % k, water_cluster(k, rng)))
+    return out

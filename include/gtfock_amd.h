@@ -118,17 +118,29 @@ int gtf_batch_class_work(gtf_batch_t batch, long *nquartets, double *nprimquarte
  * ediff.  energy[m] = final total energy, iters[m] = builds used (may be
  * NULL), history[m * max_iters + it] = energy of build it (may be NULL). */
 int gtf_batch_scf(gtf_batch_t batch, int max_iters, double ediff, double *energy, int *iters, double *history);
+/* gtf_batch_scf that calls done(ctx, m, energy, builds) on the calling
+ * thread as soon as molecule m's SCF ends (converged, diverged with builds
+ * < 0, or out of iterations); done may be NULL */
+typedef void (*gtf_scf_done_fn)(void *ctx, int m, double energy, int builds);
+int gtf_batch_scf_cb(gtf_batch_t batch, int max_iters, double ediff, double *energy, int *iters, double *history,
+                     gtf_scf_done_fn done, void *ctx);
 /* wall-clock time (gettimeofday seconds) at which each molecule's last
  * gtf_batch_scf ended; returns nmol */
 int gtf_batch_done_times(gtf_batch_t batch, double *t);
+/* estimated device bytes of one molecule in a batch running SCF */
+double gtf_batch_job_bytes(BasisSet_t basis);
 
 /* ---- an SCF request server --------------------------------------------
  *
- * Jobs queue on submission; a scheduler admits them in arrival order into
- * cohorts that fit the device-memory budget (footprint estimated from the
- * basis), at most max_jobs_per_cohort each, and runs every cohort as one
- * batch (gtf_batch_scf) on its own thread and streams, up to max_cohorts at
- * once.  The BasisSet of a job must stay alive until the job is done. */
+ * Jobs queue on submission.  A scheduler admits them in arrival order into
+ * cohorts: the oldest queued job opens a cohort and takes the queued jobs
+ * of its size class (basis functions within the same power-of-two band),
+ * oldest first, up to max_jobs_per_cohort and the device-memory budget
+ * (gtf_batch_job_bytes per job).  Each cohort runs as one batch
+ * (gtf_batch_scf_cb) on its own thread and streams, up to max_cohorts at
+ * once; a job's result is published when its own SCF ends, its memory
+ * returns to the budget when its cohort ends.  The BasisSet of a job must
+ * stay alive until the job is done. */
 typedef struct GtfServer *gtf_server_t;
 
 int gtf_server_create(double hbm_budget_bytes, int max_cohorts, int max_jobs_per_cohort, double tolscr, double tau,

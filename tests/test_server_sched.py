@@ -1,0 +1,118 @@
+"""Host tests of the request server's scheduling (gtfock_amd/csrc/serve.cpp)
+built against a stand-in batch backend (tests/sched_stub.cpp): size-class
+cohorts, per-job publication, the memory budget, the cohort limit, jobs
+that can never fit, and cancellation.  No GPU is used."""
+import ctypes
+import os
+import subprocess
+import time
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+
+@pytest.fixture(scope="module")
+def lib(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("sched") / "libsched.so")
+    subprocess.check_call(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-o", out,
+                           os.path.join(ROOT, "gtfock_amd", "csrc", "serve.cpp"), os.path.join(HERE, "sched_stub.cpp"),
+                           "-lpthread"])
+    L = ctypes.CDLL(out)
+    vp, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)
+    L.gtf_server_create.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_int, ctypes.c_double, ctypes.POINTER(vp)]
+    L.gtf_server_submit.argtypes = [vp, vp]
+    L.gtf_server_submit.restype = ctypes.c_long
+    L.gtf_server_poll.argtypes = [vp, ctypes.c_long, dp]
+    L.gtf_server_wait.argtypes = [vp]
+    L.gtf_server_stats.argtypes = [vp, dp]
+    L.gtf_server_destroy.argtypes = [vp]
+    L.stub_molecule.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    L.stub_molecule.restype = vp
+    L.stub_cohort_sizes.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    return L
+
+
+def server(L, budget, cohorts, jobs_per_cohort):
+    h = ctypes.c_void_p()
+    assert L.gtf_server_create(budget, cohorts, jobs_per_cohort, 1e-11, 0.0, 30, 1e-8, ctypes.byref(h)) == 0
+    return h
+
+
+def poll(L, h, i):
+    out = (ctypes.c_double * 7)()
+    st = L.gtf_server_poll(h, i, out)
+    return st, list(out)
+
+
+def stats(L, h):
+    out = (ctypes.c_double * 8)()
+    L.gtf_server_stats(h, out)
+    return dict(zip(["jobs", "queued", "running", "cohorts", "peak_running", "bytes", "peak_bytes", "budget"], out))
+
+
+def test_size_classes_and_early_results(lib):
+    L = lib
+    L.stub_reset()
+    h = server(L, 1e12, 4, 64)
+    small = [L.stub_molecule(30, 0.05, -1.0 - k) for k in range(6)]
+    big = [L.stub_molecule(800, 0.6, -100.0 - k) for k in range(2)]
+    ids = [L.gtf_server_submit(h, b) for b in big[:1] + small + big[1:]]
+    time.sleep(0.3)
+    # the small jobs, in a cohort of their own, are done while the large ones run
+    sts = [poll(L, h, i)[0] for i in ids]
+    assert sts[1:7] == [2] * 6
+    assert sts[0] in (0, 1) and sts[7] in (0, 1)
+    L.gtf_server_wait(h)
+    res = [poll(L, h, i) for i in ids]
+    assert all(st == 2 for st, _ in res)
+    assert [r[0] for _, r in res] == [-100.0] + [-1.0 - k for k in range(6)] + [-101.0]
+    sizes = (ctypes.c_int * 16)()
+    n = L.stub_cohort_sizes(sizes, 16)
+    # molecules of one size class share cohorts; none mixes 30 and 800 functions
+    assert sum(sizes[:n]) == 8
+    cohort_of = {i: int(poll(L, h, i)[1][5]) for i in ids}
+    assert not ({cohort_of[i] for i in ids[1:7]} & {cohort_of[ids[0]], cohort_of[ids[7]]})
+    L.gtf_server_destroy(h)
+
+
+def test_budget_and_cohort_limit(lib):
+    L = lib
+    L.stub_reset()
+    one = 1000.0 * 100 * 100
+    h = server(L, 2.5 * one, 2, 64)  # room for two jobs at a time
+    mols = [L.stub_molecule(100, 0.05, -float(k)) for k in range(7)]
+    ids = [L.gtf_server_submit(h, b) for b in mols]
+    L.gtf_server_wait(h)
+    st = stats(L, h)
+    assert all(poll(L, h, i)[0] == 2 for i in ids)
+    assert st["peak_bytes"] <= st["budget"]
+    assert st["peak_running"] <= 2 and L.stub_max_alive() <= 2
+    assert st["bytes"] == 0.0
+    L.gtf_server_destroy(h)
+
+
+def test_job_larger_than_budget_fails_alone(lib):
+    L = lib
+    L.stub_reset()
+    h = server(L, 1000.0 * 200 * 200, 4, 64)
+    ok = L.stub_molecule(100, 0.02, -1.0)
+    huge = L.stub_molecule(400, 0.02, -2.0)
+    ids = [L.gtf_server_submit(h, huge), L.gtf_server_submit(h, ok)]
+    L.gtf_server_wait(h)
+    assert poll(L, h, ids[0])[0] == -2
+    assert poll(L, h, ids[1])[0] == 2
+    L.gtf_server_destroy(h)
+
+
+def test_destroy_cancels_queued_jobs(lib):
+    L = lib
+    L.stub_reset()
+    h = server(L, 1e12, 1, 1)  # one job at a time
+    mols = [L.stub_molecule(50, 0.2, -1.0) for _ in range(4)]
+    ids = [L.gtf_server_submit(h, b) for b in mols]
+    time.sleep(0.05)
+    L.gtf_server_destroy(h)  # waits for the running job, cancels the rest
+    assert L.stub_created() == 1
