@@ -117,6 +117,10 @@ def lib():
         "gtf_pfock_device_bytes": ([_vp], ctypes.c_double),
         "gtf_erd_quartets": ([_vp, ctypes.c_int, _ip, _dp], ctypes.c_int),
         "gtf_dgemm_host": ([ctypes.c_int, _dp, _dp, _dp, ctypes.c_int], ctypes.c_int),
+        "gtf_gemm_host": ([ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "gtf_purif_set_precision": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+        "gtf_purif_fp32_iters": ([_vp], ctypes.c_int),
+        "gtf_batch_set_purif_precision": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "create_purif": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], _vp),
         "destroy_purif": ([_vp], None),
         "compute_purification": ([_vp, _dp, _dp], ctypes.c_int),
@@ -371,13 +375,14 @@ class ERD:
         return [out[offs[i]:offs[i + 1]] for i in range(len(q))]
 
 
-def dgemm(A, B, transB=False):
-    """FP64 MFMA GEMM on the GPU: A @ B, or A @ B.T when transB."""
+def dgemm(A, B, transB=False, fp32=False):
+    """MFMA GEMM on the GPU: A @ B, or A @ B.T when transB; FP64, or FP32
+    on rounded operands when fp32."""
     A = np.ascontiguousarray(A, dtype=np.float64)
     B = np.ascontiguousarray(B, dtype=np.float64)
     n = A.shape[0]
     C = np.zeros((n, n))
-    _check(lib().gtf_dgemm_host(n, _p(A), _p(B), _p(C), 1 if transB else 0), "gtf_dgemm_host")
+    _check(lib().gtf_gemm_host(n, _p(A), _p(B), _p(C), 1 if transB else 0, 1 if fp32 else 0), "gtf_gemm_host")
     return C
 
 
@@ -411,6 +416,13 @@ class Purif:
         F = np.array(F, dtype=np.float64, order="C")
         lib().compute_diis(pfock.h if pfock is not None else None, self.h, _p(D), _p(F), it)
         return F
+
+    def set_precision(self, fp32, fp32_until=1e-3):
+        """Mixed-precision purification products (gtf_purif_set_precision)."""
+        _check(lib().gtf_purif_set_precision(self.h, 1 if fp32 else 0, fp32_until), "gtf_purif_set_precision")
+
+    def fp32_iters(self):
+        return lib().gtf_purif_fp32_iters(self.h)
 
     def purify(self, F):
         F = np.ascontiguousarray(F, dtype=np.float64)
@@ -463,6 +475,10 @@ class Batch:
 
     def use_graph(self, on):
         self._st(lib().gtf_batch_use_graph(self.h, 1 if on else 0), "gtf_batch_use_graph")
+
+    def set_purif_precision(self, fp32, fp32_until=1e-3):
+        self._st(lib().gtf_batch_set_purif_precision(self.h, 1 if fp32 else 0, fp32_until),
+                 "gtf_batch_set_purif_precision")
 
     def pack(self, mats):
         out = np.zeros(self.total)
@@ -584,15 +600,18 @@ class PurifT(ctypes.Structure):
         ("timepass", ctypes.c_double), ("dev", _vp)]
 
 
-def scf(gbs, xyz, niters, tolscr=1e-11, verbose=False, tau=0.0):
+def scf(gbs, xyz, niters, tolscr=1e-11, verbose=False, tau=0.0, purif_fp32=False):
     """pscf/scf.c:189-474 on one GPU with a zero initial density
     (core-Hamiltonian guess).  tau > 0 selects the mixed-precision Fock
-    build.  Returns (energies, last D, last F = H + G(D))."""
+    build, purif_fp32 the mixed-precision purification.  Returns
+    (energies, last D, last F = H + G(D))."""
     basis = BasisSet(gbs, xyz)
     pf = PFock(basis, tolscr=tolscr)
     if tau > 0:
         pf.set_precision(tau)
     pu = Purif(basis)
+    if purif_fp32:
+        pu.set_precision(True)
     S = pf.overlap()
     X = pf.x_matrix()
     H = pf.core_hamiltonian()

@@ -78,7 +78,12 @@ struct GtfBatch {
     std::vector<double> done_time;  // wall clock at which each molecule's SCF ended
     double time_fock = 0, time_diis = 0, time_purif = 0;
 
-    BatchLayout layout() const { return BatchLayout{nmol, maxn, maxpld, d_off, d_n, d_poff, d_pld}; }
+    double *d_part = nullptr, *d_pcoef = nullptr;
+    // FP32 MFMA purification products while ||D - D^2||_F > purif_fp32_until
+    bool purif_fp32 = false;
+    double purif_fp32_until = 1e-3;
+    int* d_fp32 = nullptr;
+    BatchLayout layout() const { return BatchLayout{nmol, maxn, maxpld, d_off, d_n, d_poff, d_pld, d_part, d_pcoef}; }
     int enqueue_fock();
     int fock();
     int scf_setup();
@@ -190,6 +195,10 @@ int GtfBatch::scf_setup()
     d_skip = s.dalloc<int>(nmol);
     d_nvec = s.dalloc<int>(nmol);
     d_slot = s.dalloc<int>(nmol);
+    d_part = s.dalloc<double>((size_t)nmol * BPART * BMAX_DOTS);
+    d_pcoef = s.dalloc<double>(2 * (size_t)nmol);
+    d_fp32 = s.dalloc<int>(nmol);
+    if (!d_part || !d_pcoef || !d_fp32) return -2;
     if (!d_H || !d_S || !d_F || !T || !V || !evecs || !Fvecs || !d_stats || !d_dots || !d_coef || !d_energy ||
         !d_iters || !d_skip || !d_nvec || !d_slot)
         return -2;
@@ -371,14 +380,17 @@ int GtfBatch::purify(const std::vector<int>& active)
     const int* skip = d_skip;
     if (gtf_bpurif_init(stream, L, d_nocc, skip, p_F, p_D)) return -1;
     if (gtf_fill_int(stream, d_iters, nmol, -1)) return -1;
+    if (gtf_fill_int(stream, d_fp32, nmol, purif_fp32 ? 1 : 0)) return -1;
     std::vector<int> it_h(nmol, -1);
     int it = 0;
     for (it = 0; it < 200; it++) {
         // molecules already converged skip their GEMMs through d_iters >= 0
-        if (gtf_bgemm(stream, L, skip, d_iters, p_D, p_D, p_D2, 0) ||
-            gtf_bgemm(stream, L, skip, d_iters, p_D, p_D2, p_D3, 0))
+        if (gtf_bgemm(stream, L, skip, d_iters, p_D, p_D, p_D2, 0, d_fp32) ||
+            gtf_bgemm(stream, L, skip, d_iters, p_D, p_D2, p_D3, 0, d_fp32))
             return -1;
-        if (gtf_bpurif_stats(stream, L, skip, p_D, p_D2, p_D3, d_iters, d_stats, it, 1e-11)) return -1;
+        if (gtf_bpurif_stats(stream, L, skip, p_D, p_D2, p_D3, d_iters, d_stats, it, 1e-11, d_fp32,
+                             purif_fp32_until))
+            return -1;
         if (gtf_bpurif_update(stream, L, skip, d_iters, d_stats, p_D, p_D2, p_D3, it)) return -1;
         if (it % 2 == 1) {
             B_CHECK(hipMemcpyAsync(it_h.data(), d_iters, sizeof(int) * nmol, hipMemcpyDeviceToHost, stream));
@@ -491,6 +503,14 @@ int gtf_batch_set_precision(gtf_batch_t b, double tau)
     b->sys.tau = tau;
     b->invalidate_graph();
     return b->sys.build_tiles();
+}
+
+int gtf_batch_set_purif_precision(gtf_batch_t b, int fp32, double fp32_until)
+{
+    if (!b || fp32_until <= 0.0) return -3;
+    b->purif_fp32 = fp32 != 0;
+    b->purif_fp32_until = fp32_until;
+    return 0;
 }
 
 int gtf_batch_use_graph(gtf_batch_t b, int on)

@@ -7,6 +7,8 @@
 
 // per-molecule DIIS dot products / coefficients (MAX_DIIS + 1 rounded up)
 #define BMAX_DOTS 12
+// workgroups per molecule of the reductions
+#define BPART 64
 
 // device arrays describing the two per-molecule layouts
 struct BatchLayout {
@@ -16,16 +18,21 @@ struct BatchLayout {
     const int* n;
     const long* poff;  // padded block offsets (pld x pld)
     const int* pld;
+    double* part;      // reduction scratch, nmol * BPART * BMAX_DOTS doubles
+    double* coef;      // 2 per molecule (purification start)
 };
 
-// skip[m] != 0 or done[m] >= 0 (either array may be null) leaves molecule m alone
+// skip[m] != 0 or done[m] >= 0 (either array may be null) leaves molecule m
+// alone; fp32[m] != 0 (fp32 may be null) multiplies its operands in FP32
 int gtf_bgemm(hipStream_t st, const BatchLayout& L, const int* skip, const int* done, const double* A,
-              const double* B, double* C, int transB);
+              const double* B, double* C, int transB, const int* fp32 = nullptr);
 int gtf_bpad(hipStream_t st, const BatchLayout& L, const int* skip, const double* packed, double* padded);
 int gtf_bunpad(hipStream_t st, const BatchLayout& L, const int* skip, const double* padded, double* packed);
 int gtf_bpurif_init(hipStream_t st, const BatchLayout& L, const int* nocc, const int* skip, const double* F, double* D);
+// also clears fp32[m] (may be null) once ||D - D^2||_F < fp32_until
 int gtf_bpurif_stats(hipStream_t st, const BatchLayout& L, const int* skip, const double* D, const double* D2,
-                     const double* D3, int* iters, double* stats, int it, double tol);
+                     const double* D3, int* iters, double* stats, int it, double tol, int* fp32 = nullptr,
+                     double fp32_until = 0.0);
 int gtf_bpurif_update(hipStream_t st, const BatchLayout& L, const int* skip, const int* iters, const double* stats,
                       double* D, const double* D2, const double* D3, int it);
 int gtf_bantisym(hipStream_t st, const BatchLayout& L, const int* skip, const double* W, double* E);

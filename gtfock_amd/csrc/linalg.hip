@@ -2,87 +2,43 @@
 // deterministic reductions used by purification and DIIS.
 //
 // GEMM: C = A * B or C = A * B^T for row-major matrices of order n padded to
-// ld (a multiple of 64, zero padding).  A 256-thread workgroup computes a
-// 64 x 64 tile; each of its four waves owns a 32 x 32 quadrant as 2 x 2
-// tiles of v_mfma_f64_16x16x4_f64, fed from LDS tiles of depth 16.
+// ld (a multiple of 64, zero padding), one 64 x 64 tile per workgroup
+// (gemm_tile.h), FP64 MFMA or FP32 MFMA on rounded operands.
 #include "gtf_internal.h"
 #include "linalg.h"
 #include "screen.h"
 
-typedef double double4_t __attribute__((ext_vector_type(4)));
+#include "gemm_tile.h"
 
-#define GT 64   // C tile
-#define KT 16   // k depth per LDS stage
-#define LDS_PAD 1
-
-template <int TRANSB>
+template <int TRANSB, bool F32>
 __global__ __launch_bounds__(256) void dgemm_mfma_kernel(int ld, const double* __restrict__ A,
                                                          const double* __restrict__ B, double* __restrict__ C)
 {
-    __shared__ double As[GT][KT + LDS_PAD];
-    __shared__ double Bs[KT][GT + LDS_PAD];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int row0 = blockIdx.y * GT, col0 = blockIdx.x * GT;
-    double4_t acc[2][2];
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
-    const int li = lane & 15, lk = lane >> 4;
-    for (int k0 = 0; k0 < ld; k0 += KT) {
-        // A tile 64 x 16: thread loads rows (tid/16) + 16*r, column tid%16
-        for (int r = 0; r < 4; r++) {
-            const int rr = (tid >> 4) + 16 * r, kk = tid & 15;
-            As[rr][kk] = A[(long)(row0 + rr) * ld + k0 + kk];
-        }
-        if (TRANSB) {
-            // Bs[kk][jj] = B[col0 + jj][k0 + kk]
-            for (int r = 0; r < 4; r++) {
-                const int jj = (tid >> 4) + 16 * r, kk = tid & 15;
-                Bs[kk][jj] = B[(long)(col0 + jj) * ld + k0 + kk];
-            }
-        } else {
-            for (int r = 0; r < 4; r++) {
-                const int kk = (tid >> 6) + 4 * r, jj = tid & 63;
-                Bs[kk][jj] = B[(long)(k0 + kk) * ld + col0 + jj];
-            }
-        }
-        __syncthreads();
-        for (int kk = 0; kk < KT; kk += 4) {
-            double a[2], b[2];
-            for (int t = 0; t < 2; t++) {
-                a[t] = As[wr * 32 + t * 16 + li][kk + lk];
-                b[t] = Bs[kk + lk][wc * 32 + t * 16 + li];
-            }
-            for (int i = 0; i < 2; i++)
-                for (int j = 0; j < 2; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
-        __syncthreads();
+    gemm_tile<TRANSB, F32>(ld, A, B, C, blockIdx.y * GEMM_GT, blockIdx.x * GEMM_GT);
+}
+
+int gtf_gemm(hipStream_t st, int n, int ld, const double* A, const double* B, double* C, int transB, int fp32)
+{
+    (void)n;
+    if (ld % GEMM_GT != 0) {
+        fprintf(stderr, "gtf_gemm: leading dimension %d is not a multiple of %d\n", ld, GEMM_GT);
+        return -1;
     }
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++)
-            for (int r = 0; r < 4; r++) {
-                // v_mfma_f64_16x16x4_f64 result: lane l, register r holds
-                // row l/16 + 4r, column l%16 of the 16 x 16 tile
-                const int row = row0 + wr * 32 + i * 16 + lk + 4 * r;
-                const int col = col0 + wc * 32 + j * 16 + li;
-                C[(long)row * ld + col] = acc[i][j][r];
-            }
+    dim3 grid(ld / GEMM_GT, ld / GEMM_GT);
+    if (fp32) {
+        if (transB) dgemm_mfma_kernel<1, true><<<grid, 256, 0, st>>>(ld, A, B, C);
+        else dgemm_mfma_kernel<0, true><<<grid, 256, 0, st>>>(ld, A, B, C);
+    } else {
+        if (transB) dgemm_mfma_kernel<1, false><<<grid, 256, 0, st>>>(ld, A, B, C);
+        else dgemm_mfma_kernel<0, false><<<grid, 256, 0, st>>>(ld, A, B, C);
+    }
+    GTF_CHECK(hipGetLastError());
+    return 0;
 }
 
 int gtf_dgemm(hipStream_t st, int n, int ld, const double* A, const double* B, double* C, int transB)
 {
-    (void)n;
-    if (ld % GT != 0) {
-        fprintf(stderr, "gtf_dgemm: leading dimension %d is not a multiple of %d\n", ld, GT);
-        return -1;
-    }
-    dim3 grid(ld / GT, ld / GT);
-    if (transB) dgemm_mfma_kernel<1><<<grid, 256, 0, st>>>(ld, A, B, C);
-    else dgemm_mfma_kernel<0><<<grid, 256, 0, st>>>(ld, A, B, C);
-    GTF_CHECK(hipGetLastError());
-    return 0;
+    return gtf_gemm(st, n, ld, A, B, C, transB, 0);
 }
 
 // ---- elementwise ------------------------------------------------------------
@@ -363,7 +319,14 @@ size_t gtf_partial_bytes() { return sizeof(double) * NPART * MAX_DOTS; }
 
 // host-array GEMM for tests: C = A * B (transB = 0) or A * B^T (transB = 1),
 // n x n row-major, padded internally to a multiple of 64
+extern "C" int gtf_gemm_host(int n, const double* A, const double* B, double* C, int transB, int fp32);
+
 extern "C" int gtf_dgemm_host(int n, const double* A, const double* B, double* C, int transB)
+{
+    return gtf_gemm_host(n, A, B, C, transB, 0);
+}
+
+extern "C" int gtf_gemm_host(int n, const double* A, const double* B, double* C, int transB, int fp32)
 {
     const int ld = ((n + 63) / 64) * 64;
     const size_t m = (size_t)ld * ld;
@@ -378,7 +341,7 @@ extern "C" int gtf_dgemm_host(int n, const double* A, const double* B, double* C
         hipMemcpy2D(dA, ld * 8, A, n * 8, n * 8, n, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy2D(dB, ld * 8, B, n * 8, n * 8, n, hipMemcpyHostToDevice) != hipSuccess)
         rc = -1;
-    if (!rc) rc = gtf_dgemm(nullptr, n, ld, dA, dB, dC, transB);
+    if (!rc) rc = gtf_gemm(nullptr, n, ld, dA, dB, dC, transB, fp32);
     if (!rc && (hipDeviceSynchronize() != hipSuccess ||
                 hipMemcpy2D(C, n * 8, dC, ld * 8, n * 8, n, hipMemcpyDeviceToHost) != hipSuccess))
         rc = -1;

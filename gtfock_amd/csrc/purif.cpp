@@ -27,6 +27,12 @@ struct purif_dev {
     double *W = nullptr, *W2 = nullptr, *evecs = nullptr, *Fvecs = nullptr;
     double *scal = nullptr, *partial = nullptr, *coef = nullptr;
     bool xs_loaded = false;
+    // FP32 MFMA for the purification products while ||D - D^2||_F exceeds
+    // fp32_until, FP64 from there to the end (the FP64 iterations converge to
+    // the same projector)
+    bool fp32 = false;
+    double fp32_until = 1e-3;
+    int fp32_iters = 0;
 };
 
 static double wtime()
@@ -71,11 +77,11 @@ static int load_xs(purif_t* p)
 
 // two GEMMs timed with events; adds the kernel time to *acc
 static int gemm_pair(purif_dev* d, const double* A1, const double* B1, double* C1, int t1, const double* A2,
-                     const double* B2, double* C2, int t2, double* acc)
+                     const double* B2, double* C2, int t2, double* acc, int fp32 = 0)
 {
     PD_CHECK(hipEventRecord(d->e0, d->stream));
-    if (gtf_dgemm(d->stream, d->n, d->ld, A1, B1, C1, t1)) return -1;
-    if (A2 && gtf_dgemm(d->stream, d->n, d->ld, A2, B2, C2, t2)) return -1;
+    if (gtf_gemm(d->stream, d->n, d->ld, A1, B1, C1, t1, fp32)) return -1;
+    if (A2 && gtf_gemm(d->stream, d->n, d->ld, A2, B2, C2, t2, fp32)) return -1;
     PD_CHECK(hipEventRecord(d->e1, d->stream));
     PD_CHECK(hipEventSynchronize(d->e1));
     float ms = 0.f;
@@ -179,8 +185,11 @@ static int purify(purif_t* p, double* F_block, double* D_block, int* iters)
     const double lambda = std::min((double)nocc / (hmax - mu), (double)(n - nocc) / (mu - hmin));
     if (gtf_purif_init(d->stream, d->F, d->D, n, ld, -lambda / n, lambda * mu / n + (double)nocc / n)) return -1;
     int it;
+    bool low = d->fp32;
+    d->fp32_iters = 0;
     for (it = 0; it < 200; it++) {
-        if (gemm_pair(d, d->D, d->D, d->D2, 0, d->D, d->D2, d->D3, 0, &p->timedgemm)) return -1;
+        if (gemm_pair(d, d->D, d->D, d->D2, 0, d->D, d->D2, d->D3, 0, &p->timedgemm, low ? 1 : 0)) return -1;
+        d->fp32_iters += low ? 1 : 0;
         const double t0 = wtime();
         if (gtf_purif_stats(d->stream, d->D, d->D2, d->D3, n, ld, d->partial, d->scal)) return -1;
         double s[3];
@@ -189,6 +198,7 @@ static int purify(purif_t* p, double* F_block, double* D_block, int* iters)
         const double err = sqrt(s[0]), c = s[1] / s[2];
         if (gtf_purif_update(d->stream, d->D, d->D2, d->D3, m, c)) return -1;
         p->timetr += wtime() - t0;
+        if (low && err < d->fp32_until) low = false;
         if (err < 1e-11) break;
     }
     // D = X D X (pscf/purif.c:469-476)
@@ -296,6 +306,16 @@ static int diis(purif_t* p, double* D_block, double* F_block, int iter)
     }
     return d2h(d, F_block, curF, p->ldx);
 }
+
+int gtf_purif_set_precision(purif_t* p, int fp32, double fp32_until)
+{
+    if (!p || !p->dev || fp32_until <= 0.0) return -1;
+    p->dev->fp32 = fp32 != 0;
+    p->dev->fp32_until = fp32_until;
+    return 0;
+}
+
+int gtf_purif_fp32_iters(purif_t* p) { return p && p->dev ? p->dev->fp32_iters : -1; }
 
 void compute_diis(PFock_t pfock, purif_t* p, double* D_block, double* F_block, int iter)
 {

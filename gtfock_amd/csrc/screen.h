@@ -19,3 +19,5 @@ int gtf_add(hipStream_t st, const double* A, const double* B, double* C, long n)
 // C = alpha * op(A) * op(B)^T-convention GEMM on row-major n x n matrices with leading dimension ld
 // transB: 1 -> C = A * B^T (pdgemm3D_2), 0 -> C = A * B
 int gtf_dgemm(hipStream_t st, int n, int ld, const double* A, const double* B, double* C, int transB);
+// the same with FP32 MFMA on operands rounded to FP32 when fp32 != 0
+int gtf_gemm(hipStream_t st, int n, int ld, const double* A, const double* B, double* C, int transB, int fp32);

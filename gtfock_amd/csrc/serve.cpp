@@ -111,6 +111,7 @@ void GtfServer::run_cohort(std::vector<long> ids, double mem, int cohort)
     CohortCtx ctx{this, &ids};
     gtf_batch_t b = nullptr;
     int rc = gtf_batch_create(bases.data(), nmol, tolscr, &b);
+    // the mixed-precision form: FP32 ERIs below the Schwarz bound tau
     if (rc == 0 && tau > 0.0) rc = gtf_batch_set_precision(b, tau);
     if (rc == 0) rc = gtf_batch_scf_cb(b, max_iters, ediff, e.data(), it.data(), nullptr, publish, &ctx);
     gtf_batch_destroy(b);

@@ -73,6 +73,17 @@ int gtf_erd_quartets(ERD_t erd, int nq, const int *quartets, double *out);
 /* The FP64 MFMA GEMM on host arrays (n x n, row-major): C = A * B when
  * transB = 0, C = A * B^T (the pdgemm3D_2 convention) when transB = 1. */
 int gtf_dgemm_host(int n, const double *A, const double *B, double *C, int transB);
+/* The same; fp32 != 0 rounds A and B to FP32 and uses the FP32 MFMA. */
+int gtf_gemm_host(int n, const double *A, const double *B, double *C, int transB, int fp32);
+
+/* Mixed-precision purification (compute_purification): with fp32 != 0 the
+ * D^2 and D^3 products use FP32 MFMA while ||D - D^2||_F >= fp32_until and
+ * FP64 MFMA from then on, so the FP64 iterations correct the FP32 residual
+ * and converge to the same projector; the final X D X stays FP64. */
+struct _purif_t;
+int gtf_purif_set_precision(struct _purif_t *purif, int fp32, double fp32_until);
+/* purification iterations of the last compute_purification run in FP32 */
+int gtf_purif_fp32_iters(struct _purif_t *purif);
 
 /* ---- batches: many molecules per Fock build and per SCF iteration ------
  *
@@ -94,6 +105,9 @@ int gtf_batch_layout(gtf_batch_t batch, int *nbf, long *offset);
 long gtf_batch_total(gtf_batch_t batch);
 /* FP32 ERIs for quartets with Schwarz bound below tau (0: FP64 only) */
 int gtf_batch_set_precision(gtf_batch_t batch, double tau);
+/* mixed-precision purification inside gtf_batch_scf, per molecule, as
+ * gtf_purif_set_precision */
+int gtf_batch_set_purif_precision(gtf_batch_t batch, int fp32, double fp32_until);
 /* 1 (default): replay the build as a captured HIP graph; 0: eager launches */
 int gtf_batch_use_graph(gtf_batch_t batch, int on);
 int gtf_batch_put_density(gtf_batch_t batch, int m, const double *D);

@@ -55,6 +55,7 @@ void gtf_batch_destroy(gtf_batch_t b)
 }
 
 int gtf_batch_set_precision(gtf_batch_t, double) { return 0; }
+int gtf_batch_set_purif_precision(gtf_batch_t, int, double) { return 0; }
 
 int gtf_batch_scf_cb(gtf_batch_t b, int, double, double* energy, int* iters, double*, gtf_scf_done_fn done,
                      void* ctx)

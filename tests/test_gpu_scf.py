@@ -107,3 +107,48 @@ def test_mfma_dgemm(g, n):
     B = rng.normal(size=(n, n))
     assert np.abs(g.dgemm(A, B) - A @ B).max() < 1e-12 * n
     assert np.abs(g.dgemm(A, B, transB=True) - A @ B.T).max() < 1e-12 * n
+
+
+@gpu
+@pytest.mark.parametrize("n", [64, 250, 846])
+def test_fp32_mfma_gemm(g, n):
+    rng = np.random.default_rng(n + 7)
+    A = rng.normal(size=(n, n))
+    B = rng.normal(size=(n, n))
+    for transB in (False, True):
+        ref = A.astype(np.float32).astype(np.float64) @ (B.T if transB else B).astype(np.float32).astype(np.float64)
+        C = g.dgemm(A, B, transB=transB, fp32=True)
+        # FP32 accumulation of n products of magnitude ~1
+        assert np.abs(C - ref).max() < 2e-5 * np.sqrt(n)
+
+
+@gpu
+def test_mixed_purification_projector(g, gbs, mol_dir):
+    """FP32 MFMA products until ||D - D^2|| < 1e-3, FP64 from there.  The
+    FP64 iterations restore idempotency to FP64 precision; they map an
+    occupied-virtual rotation E of the FP32 phase to PE + EP - 2PEP = E, so
+    the projector stays within FP32 resolution of the FP64 one (the SCF is
+    stationary in such rotations, see the SCF test below)."""
+    xyz = str(mol_dir / "c10h22.xyz")
+    b = g.BasisSet(gbs, xyz)
+    pf = g.PFock(b)
+    S, X, H = pf.overlap(), pf.x_matrix(), pf.core_hamiltonian()
+    pf.close()
+    p64 = g.Purif(b)
+    p64.set_xs(X, S)
+    D64, it64 = p64.purify(X @ H @ X)
+    p32 = g.Purif(b)
+    p32.set_xs(X, S)
+    p32.set_precision(True)
+    D32, it32 = p32.purify(X @ H @ X)
+    n32 = p32.fp32_iters()
+    idem = np.abs(D32 @ S @ D32 - D32).max()
+    tr = np.trace(D32 @ S)
+    print("purification: FP64 %d iterations; mixed %d (%d in FP32); max|dD| %.2e, |DSD - D| %.2e, tr DS %.12f" % (
+        it64, it32, n32, np.abs(D32 - D64).max(), idem, tr))
+    assert n32 > 0
+    assert idem < 1e-9
+    assert abs(tr - b.nocc) < 1e-9
+    assert np.abs(D32 - D64).max() < 1e-4
+    p64.close()
+    p32.close()
