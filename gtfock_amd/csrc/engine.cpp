@@ -266,6 +266,8 @@ int GtfSystem::screen(double tolscr)
     for (int t = 0; t < GTF_NPAIRTYPES; t++) {
         std::vector<int>& s = h_sig[t];
         s.clear();
+        h_listprim[t].clear();
+        h_segmax[t].clear();
         h_sigv[t].clear();
         h_rank[t].clear();
         h_sigmol[t].clear();
@@ -412,6 +414,29 @@ std::vector<double> GtfSystem::ket_prim_prefix(int B) const
     return kp;
 }
 
+// prefix sums of primitive-pair counts over a type list in list order
+const std::vector<double>& GtfSystem::list_prim_prefix(int B)
+{
+    std::vector<double>& lp = h_listprim[B];
+    if ((int)lp.size() != nsig[B] + 1) {
+        lp.assign(nsig[B] + 1, 0.0);
+        for (int j = 0; j < nsig[B]; j++) lp[j + 1] = lp[j] + h_pnprim[h_sig[B][j]];
+    }
+    return lp;
+}
+
+int GtfSystem::seg_nprim_max(int B, int sg)
+{
+    std::vector<int>& mx = h_segmax[B];
+    if ((int)mx.size() != nseg) {
+        mx.assign(nseg, 1);
+        for (int g = 0; g < nseg; g++)
+            for (int j = seg_start[B][g]; j < seg_start[B][g] + seg_len[B][g]; j++)
+                mx[g] = std::max(mx[g], h_pnprim[h_sig[B][j]]);
+    }
+    return mx[sg];
+}
+
 long GtfSystem::quartets(int* out, long cap) const
 {
     long k = 0;
@@ -488,30 +513,28 @@ int GtfSystem::build_tiles()
         const std::vector<double>& vs = h_sorted[B];
         const std::vector<double> kp = ket_prim_prefix(B);
         const int ns = ci->nsplit;
-        // tiles of the kets [b, e) of one row, cut by work units and by
-        // primitive quartets; ich = bra primitives per unit
+        // Tiles of the kets [b, e) of one row of segment sg, cut by work units
+        // and by primitive quartets.  ich (bra primitives per unit) follows the
+        // largest ket primitive count of the segment (kets of a segment share a
+        // primitive-count bucket); with it fixed, a tile's unit count grows by
+        // a constant per ket, and its primitive-quartet cost is read off the
+        // prefix sums of the list, so each tile end is one binary search.
+        const std::vector<double>& lp = list_prim_prefix(B);
         std::vector<std::pair<double, int4>> hi, lo;
-        auto cut = [&](std::vector<std::pair<double, int4>>& out, int r, int nb, long b, long e) {
+        auto cut = [&](std::vector<std::pair<double, int4>>& out, int r, int nb, int nkmax, long b, long e) {
+            int ich = std::max(1, std::min(nb, (GTF_UNIT_PQ + nkmax - 1) / nkmax));
+            // keep one ket's units within a tile
+            while (((nb + ich - 1) / ich) * ns > GTF_TILE_UNITS && ich < nb) ich++;
+            const long per_ket = (long)((nb + ich - 1) / ich) * ns;
+            const long max_kets = std::max(1L, std::min<long>(GTF_TILE_UNITS / per_ket, tile_kets(ns)));
             long j = b;
             while (j < e) {
-                const long start = j;
-                int nkmax = 0, ich = nb;
-                double cost = 0.0;
-                while (j < e) {
-                    const int nk = h_pnprim[h_sig[B][j]];
-                    const int nkm = std::max(nkmax, nk);
-                    int ic = std::max(1, std::min(nb, (GTF_UNIT_PQ + nkm - 1) / nkm));
-                    // keep one ket's units within a tile
-                    while (((nb + ic - 1) / ic) * ns > GTF_TILE_UNITS && ic < nb) ic++;
-                    const long units = (j - start + 1) * (long)((nb + ic - 1) / ic) * ns;
-                    const double c = cost + (double)nb * nk;
-                    if (j > start && (units > GTF_TILE_UNITS || c > GTF_TILE_PQ || j - start >= tile_kets(ns))) break;
-                    nkmax = nkm;
-                    ich = ic;
-                    cost = c;
-                    j++;
-                }
-                out.push_back({cost, make_int4(r, (int)start, (int)j, ich)});
+                // the most kets whose primitive quartets stay within GTF_TILE_PQ
+                const double limit = lp[j] + (double)GTF_TILE_PQ / nb;
+                long je = std::upper_bound(lp.begin() + j + 1, lp.begin() + e + 1, limit) - lp.begin() - 1;
+                je = std::max(j + 1, std::min(je, std::min(e, j + max_kets)));
+                out.push_back({(double)nb * (lp[je] - lp[j]), make_int4(r, (int)j, (int)je, ich)});
+                j = je;
             }
         };
         for (int row = 0; row < nrows[cls]; row++) {
@@ -529,10 +552,11 @@ int GtfSystem::build_tiles()
             }
             const long pad = std::min<long>(nket, ((c + GTF_KET_BLOCK - 1) / GTF_KET_BLOCK) * GTF_KET_BLOCK);
             const long pad64 = std::min<long>(nket, ((c64 + GTF_KET_BLOCK - 1) / GTF_KET_BLOCK) * GTF_KET_BLOCK);
-            cut(hi, r, nb, s0, s0 + pad64);
+            const int nkmax = seg_nprim_max(B, sg);
+            cut(hi, r, nb, nkmax, s0, s0 + pad64);
             if (c > c64) {
                 const long b0 = (c64 / GTF_KET_BLOCK) * GTF_KET_BLOCK;
-                cut(lo, r, nb, s0 + b0, s0 + pad);
+                cut(lo, r, nb, nkmax, s0 + b0, s0 + pad);
                 nq_lo[cls] += c - c64;
                 nprimq_lo[cls] += (double)nb * (kp[s0 + c] - kp[s0 + c64]);
             }

@@ -143,6 +143,10 @@ struct GtfSystem {
     // returns the count, writes at most cap
     long quartets(int* out, long cap) const;
     std::vector<double> ket_prim_prefix(int B) const;
+    const std::vector<double>& list_prim_prefix(int B);
+    int seg_nprim_max(int B, int sg);
+    std::vector<double> h_listprim[GTF_NPAIRTYPES];  // caches of the two above
+    std::vector<int> h_segmax[GTF_NPAIRTYPES];
     void release();
     void dfree(void* p);
     ~GtfSystem() { release(); }
