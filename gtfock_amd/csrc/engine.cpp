@@ -514,18 +514,28 @@ int GtfSystem::build_tiles()
         const std::vector<double> kp = ket_prim_prefix(B);
         const int ns = ci->nsplit;
         // Tiles of the kets [b, e) of one row of segment sg, cut by work units
-        // and by primitive quartets.  ich (bra primitives per unit) follows the
-        // largest ket primitive count of the segment (kets of a segment share a
-        // primitive-count bucket); with it fixed, a tile's unit count grows by
-        // a constant per ket, and its primitive-quartet cost is read off the
-        // prefix sums of the list, so each tile end is one binary search.
+        // and by primitive quartets.  nic (bra-primitive chunks per quartet)
+        // follows the largest ket primitive count of the segment (kets of a
+        // segment share a primitive-count bucket); with it fixed, a tile's unit
+        // count grows by a constant per ket, and its primitive-quartet cost is
+        // read off the prefix sums of the list, so each tile end is one binary
+        // search.
         const std::vector<double>& lp = list_prim_prefix(B);
         std::vector<std::pair<double, int4>> hi, lo;
         auto cut = [&](std::vector<std::pair<double, int4>>& out, int r, int nb, int nkmax, long b, long e) {
-            int ich = std::max(1, std::min(nb, (GTF_UNIT_PQ + nkmax - 1) / nkmax));
+            // bra-primitive chunks per quartet: about GTF_UNIT_PQ primitive
+            // quartets per unit; a power of two not above nb when one thread
+            // owns the whole bra (its lanes then reduce with shuffles)
+            const int ich = std::max(1, std::min(nb, (GTF_UNIT_PQ + nkmax - 1) / nkmax));
+            int nic = (nb + ich - 1) / ich;
+            if (ns == 1) {
+                int p2 = 1;
+                while (2 * p2 <= nic && 2 * p2 <= 64) p2 *= 2;  // a group stays in one wavefront
+                nic = p2;
+            }
             // keep one ket's units within a tile
-            while (((nb + ich - 1) / ich) * ns > GTF_TILE_UNITS && ich < nb) ich++;
-            const long per_ket = (long)((nb + ich - 1) / ich) * ns;
+            while (nic * ns > GTF_TILE_UNITS && nic > 1) nic = ns == 1 ? nic / 2 : nic - 1;
+            const long per_ket = (long)nic * ns;
             const long max_kets = std::max(1L, std::min<long>(GTF_TILE_UNITS / per_ket, tile_kets(ns)));
             long j = b;
             while (j < e) {
@@ -533,7 +543,7 @@ int GtfSystem::build_tiles()
                 const double limit = lp[j] + (double)GTF_TILE_PQ / nb;
                 long je = std::upper_bound(lp.begin() + j + 1, lp.begin() + e + 1, limit) - lp.begin() - 1;
                 je = std::max(j + 1, std::min(je, std::min(e, j + max_kets)));
-                out.push_back({(double)nb * (lp[je] - lp[j]), make_int4(r, (int)j, (int)je, ich)});
+                out.push_back({(double)nb * (lp[je] - lp[j]), make_int4(r, (int)j, (int)je, nic)});
                 j = je;
             }
         };

@@ -84,7 +84,7 @@ __device__ __forceinline__ int find_row(const long* __restrict__ off, int nrows,
 template <int LA, int LB, int LC, int LD, int NC, bool STRIP>
 __device__ __forceinline__ void digest(const EriArgs& a, int bra, int ket, int c0,
                                        const double (&acc)[NC][EriShape<LA, LB, LC, LD>::NCD], double* jmn,
-                                       double* strip, int nbf, double* jpq);
+                                       double* strip, int nbf, double* jpq, double* jmn_reg = nullptr);
 
 template <int LA, int LB, int LC, int LD, int NC, int EPI>
 __device__ __forceinline__ void eri_unit(const EriArgs& a, long u)
@@ -140,11 +140,13 @@ __device__ __forceinline__ void eri_unit(const EriArgs& a, long u)
 // K_MQ, K_NQ, which all lie in the rows of shells M and N) go to LDS
 // (jmn[NA*NB], strip[(NA+NB) * nbf], row r < NA is function fM + r, row
 // NA + r is fN + r); J_PQ goes to the ket's LDS row jpq[NCD] when one is
-// given (the threads sharing a quartet then meet in LDS), else to HBM.
+// given (the threads sharing a quartet then meet in LDS), else to HBM.  With
+// jmn_reg (a thread that owns all bra components of its quartets) J_MN is
+// summed in those registers and added to LDS once per tile.
 template <int LA, int LB, int LC, int LD, int NC, bool STRIP>
 __device__ __forceinline__ void digest(const EriArgs& a, int bra, int ket, int c0,
                                        const double (&acc)[NC][EriShape<LA, LB, LC, LD>::NCD], double* jmn,
-                                       double* strip, int nbf, double* jpq)
+                                       double* strip, int nbf, double* jpq, double* jmn_reg)
 {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr int NCD = S::NCD, ND = S::ND, NB = S::NB, NA = S::NA;
@@ -176,8 +178,12 @@ __device__ __forceinline__ void digest(const EriArgs& a, int bra, int ket, int c
             constexpr int cd = decltype(d_)::value;
             s += D[(long)(oP + cd / ND) * ld + oQ + cd % ND] * acc[c][cd];
         });
-        if constexpr (STRIP) atomicAdd(&jmn[mloc[c] * NB + nloc[c]], cJ1 * s);
-        else GTF_ATOMIC_ADD(&J[(long)(oM + mloc[c]) * ld + oN + nloc[c]], cJ1 * s);
+        if constexpr (STRIP) {
+            if (jmn_reg) jmn_reg[c] += cJ1 * s;
+            else atomicAdd(&jmn[mloc[c] * NB + nloc[c]], cJ1 * s);
+        } else {
+            GTF_ATOMIC_ADD(&J[(long)(oM + mloc[c]) * ld + oN + nloc[c]], cJ1 * s);
+        }
     });
     if (f3) {
         static_for<0, NCD>([&](auto d_) {
@@ -269,8 +275,9 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
 // quartet loop.  Dynamic LDS: (NA + NB) * nbf + NA * NB + tile_kets * NCD
 // doubles, nbf = the largest molecule's size.
 //
-// A tile is (bra row, first ket, end ket, bra-primitive chunk ich): each of
-// its quartets is split into ceil(nprim(bra) / ich) * NSPLIT work units, so
+// A tile is (bra row, first ket, end ket, nic): each of its quartets is split
+// into nic bra-primitive chunks (sizes differing by at most one) times NSPLIT
+// bra-component chunks, so
 // that a unit holds about GTF_UNIT_PQ primitive quartets however contracted
 // the pairs are, and a tile holds at most GTF_TILE_UNITS units and about
 // GTF_TILE_PQ primitive quartets.  Digestion is linear in the integrals, so
@@ -301,8 +308,8 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
         if (task >= a.ntiles) break;
         const int4 t = a.tiles[task];
         const int bra = a.bra_list[t.x];
-        const int ich = t.w;
-        const int nic = (a.pv.nprim[bra] + ich - 1) / ich;
+        const int nbp = a.pv.nprim[bra];
+        const int nic = t.w;  // bra-primitive chunks per quartet
         // strip rows are as wide as the bra's molecule (at most a.nbf)
         const int nbf = a.mld[a.pv.s1[bra]];
         const int nstrip = (NA + NB) * nbf;
@@ -312,14 +319,29 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
         const int njpq = JPQ_LDS ? (t.z - t.y) * NCD : 0;
         for (int k = threadIdx.x; k < nstrip + NA * NB + njpq; k += 256) smem[k] = 0.0;
         __syncthreads();
-        const int per_ket = nic * NSPLIT;
-        const int nunits = (t.z - t.y) * per_ket;
+        // Unit order.  NSPLIT == 1: the nic chunks of one quartet sit in
+        // neighbouring lanes (nic is a power of two, so a group never spans
+        // two wavefronts); the group adds its partial integrals with lane
+        // shuffles and its first lane digests the sum.  NSPLIT > 1: component
+        // chunk fastest, then ket, then bra-primitive chunk, so neighbouring
+        // lanes digest into different ket columns.
+        const int nkt = t.z - t.y;
+        const int nunits = nkt * nic * NSPLIT;
+        double jreg[NC];
+        static_for<0, NC>([&](auto c_) { jreg[decltype(c_)::value] = 0.0; });
         for (int u = threadIdx.x; u < nunits; u += 256) {
-            const int q = u / per_ket;
-            const int rem = u - q * per_ket;
-            const int ic = rem / NSPLIT;
-            const int c0 = (rem - ic * NSPLIT) * NC;
-            const int i0 = ic * ich, i1 = i0 + ich;
+            int q, ic, c0;
+            if constexpr (NSPLIT == 1) {
+                q = u / nic;
+                ic = u - q * nic;
+                c0 = 0;
+            } else {
+                const int rest = u / NSPLIT;
+                c0 = (u - rest * NSPLIT) * NC;
+                ic = rest / nkt;
+                q = rest - ic * nkt;
+            }
+            const int i0 = ic * nbp / nic, i1 = (ic + 1) * nbp / nic;
             const int j = t.y + q;
             if (!quartet_valid(a, t.x, j)) continue;
             // precision split on the Schwarz bound sqrt(v_bra v_ket) vs tau
@@ -328,19 +350,34 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
             double acc[NC][NCD];
             if constexpr (LOW) {
                 float accf[NC][NCD];
-                eri_block<LA, LB, LC, LD, NC, float>(a.pv, bra, ket, c0, a.boys8f + L * BOYS8_SLAB, accf, i0,
-                                                     min(i1, a.pv.nprim[bra]));
+                eri_block<LA, LB, LC, LD, NC, float>(a.pv, bra, ket, c0, a.boys8f + L * BOYS8_SLAB, accf, i0, i1);
                 static_for<0, NC>([&](auto c_) {
                     static_for<0, NCD>([&](auto d_) {
                         acc[decltype(c_)::value][decltype(d_)::value] = (double)accf[decltype(c_)::value][decltype(d_)::value];
                     });
                 });
             } else {
-                eri_block<LA, LB, LC, LD, NC, double>(a.pv, bra, ket, c0, a.boys8 + L * BOYS8_SLAB, acc, i0,
-                                                      min(i1, a.pv.nprim[bra]));
+                eri_block<LA, LB, LC, LD, NC, double>(a.pv, bra, ket, c0, a.boys8 + L * BOYS8_SLAB, acc, i0, i1);
+            }
+            if constexpr (NSPLIT == 1) {
+                // sum the group's partial integrals into its first lane
+                for (int w = 1; w < nic; w <<= 1)
+                    static_for<0, NC>([&](auto c_) {
+                        static_for<0, NCD>([&](auto d_) {
+                            constexpr int c = decltype(c_)::value, cd = decltype(d_)::value;
+                            acc[c][cd] += __shfl_xor(acc[c][cd], w, 64);
+                        });
+                    });
+                if (ic != 0) continue;
             }
             digest<LA, LB, LC, LD, NC, true>(a, bra, ket, c0, acc, jmn, strip, nbf,
-                                             JPQ_LDS ? jpq + q * NCD : nullptr);
+                                             JPQ_LDS ? jpq + q * NCD : nullptr, NSPLIT == 1 ? jreg : nullptr);
+        }
+        if constexpr (NSPLIT == 1) {
+            static_for<0, NC>([&](auto c_) {
+                constexpr int c = decltype(c_)::value;
+                if (jreg[c] != 0.0) atomicAdd(&jmn[(c / NB) * NB + c % NB], jreg[c]);
+            });
         }
         __syncthreads();
         const int M = a.pv.s1[bra], N = a.pv.s2[bra];
