@@ -334,6 +334,30 @@ def main():
                 "algorithmic_flops_per_launch": dom_fl, "avg_launch_ms": dom_s * 1e3,
                 "build_tflops": total_fl / (ms_step * 1e-3) / 1e12}
 
+    # the same kernels each alone on the GPU (class launches serialised), after
+    # the timed region: the per-kernel efficiency without sharing the device
+    pf.set_serial(True)
+    pf.set_timing(True)
+    for _ in range(max(2, args.steps // 4)):
+        pf.compute_fock()
+    ms1, n1 = pf.class_times()
+    pf.set_timing(False)
+    pf.set_serial(False)
+    iso = []
+    for c in range(21):
+        if n1[c]:
+            t1 = ms1[c] / n1[c] * 1e-3
+            iso.append((ms1[c], c, t1, flops.class_flops(c, nq[c], npq[c])))
+    iso.sort(reverse=True)
+    _, idom, idom_s, idom_fl = iso[0]
+    roofline["isolated"] = {
+        "note": "class kernels launched one after another, same density, after the timed region",
+        "kernel": pf.class_kernel(idom), "class": names[idom], "avg_launch_ms": idom_s * 1e3,
+        "achieved": idom_fl / idom_s / 1e12, "frac": idom_fl / idom_s / FP64_PEAK,
+        "dominant_kernel_same_as_timed": idom == dom,
+        "build_ms_serial": sum(x[2] for x in iso) * 1e3,
+        "build_tflops_serial": total_fl / sum(x[2] for x in iso) / 1e12}
+
     mixed = None
     if not args.no_mixed:
         # mixed precision on the same molecule and density: FP32 ERIs for quartets

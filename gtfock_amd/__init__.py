@@ -111,6 +111,7 @@ def lib():
         "gtf_pfock_class_info": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
         "gtf_pfock_class_work": ([_vp, _lp, _dp], ctypes.c_int),
         "gtf_pfock_set_timing": ([_vp, ctypes.c_int], ctypes.c_int),
+        "gtf_pfock_set_serial": ([_vp, ctypes.c_int], ctypes.c_int),
         "gtf_pfock_set_precision": ([_vp, ctypes.c_double], ctypes.c_int),
         "gtf_pfock_mixed_counts": ([_vp, _lp, _dp], ctypes.c_int),
         "gtf_pfock_class_times": ([_vp, _dp, _lp], ctypes.c_int),
@@ -328,6 +329,10 @@ class PFock:
         if info[6] > 0:
             return "eri_jk_tile_kernel<%d, %d, %d, %d, %d, double>" % (la, lb, lc, ld, nc)
         return "eri_kernel<%d, %d, %d, %d, %d, 0>" % (la, lb, lc, ld, nc)
+
+    def set_serial(self, on):
+        """Class kernels one after another (per-kernel measurement)."""
+        _check(lib().gtf_pfock_set_serial(self.h, 1 if on else 0), "gtf_pfock_set_serial")
 
     def class_times(self):
         ms = np.zeros(21)

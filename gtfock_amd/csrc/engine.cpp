@@ -181,6 +181,11 @@ int GtfSystem::init(const std::vector<const BasisSet*>& mols, hipStream_t st)
     return 0;
 }
 
+bool GtfSystem::exchanged(int cls) const
+{
+    return gtf_eri_class(cls)->nsplit_t > 0 && ((exchanged_mask >> cls) & 1u);
+}
+
 void GtfSystem::dfree(void* p)
 {
     if (!p) return;
@@ -206,6 +211,7 @@ int GtfSystem::screen(double tolscr)
 {
     if (const char* s = getenv("GTF_JK_TILES")) use_tiles = atoi(s) != 0;
     if (const char* s = getenv("GTF_JK_SERIAL")) serial = atoi(s) != 0;
+    if (const char* s = getenv("GTF_JK_EXCHANGED")) exchanged_mask = (unsigned)strtoul(s, nullptr, 0);
     {
         int dev = 0, lds = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
@@ -501,8 +507,9 @@ int GtfSystem::build_tiles()
         const EriClassInfo* ci = gtf_eri_class(cls);
         const int NA = 2 * ci->la + 1, NB = 2 * ci->lb + 1;
         const int NCD = (2 * ci->lc + 1) * (2 * ci->ld + 1);
-        tile_lds[cls] = sizeof(double) * ((size_t)(NA + NB) * nbf + NA * NB +
-                                          (size_t)tile_kets(ci->nsplit) * NCD);
+        // threads per quartet of the kernel that runs the class
+        const int ns = exchanged(cls) ? ci->nsplit_t : ci->nsplit;
+        tile_lds[cls] = sizeof(double) * ((size_t)(NA + NB) * nbf + NA * NB + (size_t)tile_kets(ns) * NCD);
         ntiles[cls] = ntiles_lo[cls] = 0;
         nq_lo[cls] = 0;
         nprimq_lo[cls] = 0.0;
@@ -512,7 +519,6 @@ int GtfSystem::build_tiles()
         if (nq[cls] == 0 || !use_tiles || tile_lds[cls] > max_lds) continue;
         const std::vector<double>& vs = h_sorted[B];
         const std::vector<double> kp = ket_prim_prefix(B);
-        const int ns = ci->nsplit;
         // Tiles of the kets [b, e) of one row of segment sg, cut by work units
         // and by primitive quartets.  nic (bra-primitive chunks per quartet)
         // follows the largest ket primitive count of the segment (kets of a
@@ -521,13 +527,15 @@ int GtfSystem::build_tiles()
         // read off the prefix sums of the list, so each tile end is one binary
         // search.
         const std::vector<double>& lp = list_prim_prefix(B);
+        const bool exch = exchanged(cls);
         std::vector<std::pair<double, int4>> hi, lo;
         auto cut = [&](std::vector<std::pair<double, int4>>& out, int r, int nb, int nkmax, long b, long e) {
             // bra-primitive chunks per quartet: about GTF_UNIT_PQ primitive
             // quartets per unit; a power of two not above nb when one thread
-            // owns the whole bra (its lanes then reduce with shuffles)
+            // owns the whole bra (its lanes then reduce with shuffles); none
+            // for the exchanged kernels
             const int ich = std::max(1, std::min(nb, (GTF_UNIT_PQ + nkmax - 1) / nkmax));
-            int nic = (nb + ich - 1) / ich;
+            int nic = exch ? 1 : (nb + ich - 1) / ich;
             if (ns == 1) {
                 int p2 = 1;
                 while (2 * p2 <= nic && 2 * p2 <= 64) p2 *= 2;  // a group stays in one wavefront
@@ -649,17 +657,21 @@ int GtfSystem::jk(const double* D, double* J, double* K)
         if (ntiles[cls] > 0 || ntiles_lo[cls] > 0) {
             a.nbf = nbf;
             a.tau2 = tau * tau;
+            const bool exch = exchanged(cls);
             if (ntiles[cls] > 0) {
                 a.tiles = d_tiles[cls];
                 a.ntiles = ntiles[cls];
                 a.counter = d_counters + cls;
-                if (launch(ci->jk_tile, a, std::min(a.ntiles, 2048), cs, tile_lds[cls])) return -1;
+                if (launch(exch ? ci->jk_tile_t : ci->jk_tile, a, std::min(a.ntiles, 2048), cs, tile_lds[cls]))
+                    return -1;
             }
             if (ntiles_lo[cls] > 0) {
                 a.tiles = d_tiles_lo[cls];
                 a.ntiles = ntiles_lo[cls];
                 a.counter = d_counters + GTF_NCLASSES + cls;
-                if (launch(ci->jk_tile_lo, a, std::min(a.ntiles, 2048), cs, tile_lds[cls])) return -1;
+                if (launch(exch ? ci->jk_tile_t_lo : ci->jk_tile_lo, a, std::min(a.ntiles, 2048), cs,
+                           tile_lds[cls]))
+                    return -1;
             }
         } else if (launch(ci->jk, a, std::min(a.ntasks, 2048), cs)) {
             return -1;

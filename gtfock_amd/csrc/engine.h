@@ -96,6 +96,13 @@ struct GtfSystem {
     size_t tile_lds[GTF_NCLASSES] = {0};
     size_t max_lds = 65536;
     bool use_tiles = true;
+    // classes run by the bra/ket-exchanged tile kernel (bit = class index;
+    // only classes that have one).  Default: the two classes where it was
+    // measured faster on MI355X, (pp|ps) and (dp|ps); in the others the large
+    // pair's coefficients move into the inner primitive loop and the kernel
+    // loses more than the shared Boys/R work saves.
+    unsigned exchanged_mask = (1u << 4) | (1u << 11);
+    bool exchanged(int cls) const;
     // mixed precision: Schwarz bound threshold tau (0 = plain FP64)
     double tau = 0.0;
     long nq_lo[GTF_NCLASSES] = {0};

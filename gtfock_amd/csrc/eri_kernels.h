@@ -411,6 +411,163 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
     }
 }
 
+// update_F digestion of a quartet evaluated with bra and ket exchanged: the
+// thread holds (cd | ab) for its NCT ket components cd0.. and every bra
+// component ab, acc[k][ab].  Same contributions as digest<..., true>: J_MN
+// summed in jmn_reg[NAB] (registers, all bra components), J_PQ to the ket's
+// LDS row, the exchange blocks K_MP, K_NP, K_MQ, K_NQ to the bra rows of the
+// LDS strip.
+template <int LA, int LB, int LC, int LD, int NCT>
+__device__ __forceinline__ void digest_t(const EriArgs& a, int bra, int ket, int cd0,
+                                         const double (&acc)[NCT][EriShape<LA, LB, LC, LD>::NAB],
+                                         double* jmn_reg, double* strip, int nbf, double* jpq)
+{
+    using S = EriShape<LA, LB, LC, LD>;
+    constexpr int NAB = S::NAB, NA = S::NA, NB = S::NB, ND = S::ND;
+    const int M = a.pv.s1[bra], N = a.pv.s2[bra], P = a.pv.s1[ket], Q = a.pv.s2[ket];
+    const int f1 = (M != N), f2 = (P != Q), f3 = (bra != ket);
+    const double cJ1 = 2.0 * (1 + f1) * (1 + f2);
+    const double cJ2 = 2.0 * f3 * (1 + f1) * (1 + f2);
+    const double cMP = (1 + f3), cNP = f1 * (1 + f3), cMQ = f2 * (1 + f3), cNQ = f1 * f2 * (1 + f3);
+    const int ld = a.mld[M];
+    const long base = a.mbase[M];
+    const int oM = a.fstart[M], oN = a.fstart[N], oP = a.fstart[P], oQ = a.fstart[Q];
+    const double* __restrict__ D = a.D + base;
+    static_for<0, NCT>([&](auto k_) {
+        constexpr int k = decltype(k_)::value;
+        const int cd = cd0 + k;
+        const int p = cd / ND, q = cd - (cd / ND) * ND;
+        const double dpq = D[(long)(oP + p) * ld + oQ + q];
+        // J_MN += cJ1 D_pq I  and  J_PQ += cJ2 sum_mn D_mn I
+        double jpq_s = 0.0;
+        static_for<0, NAB>([&](auto ab_) {
+            constexpr int ab = decltype(ab_)::value;
+            constexpr int m = ab / NB, n = ab % NB;
+            jmn_reg[ab] += cJ1 * dpq * acc[k][ab];
+            jpq_s += D[(long)(oM + m) * ld + oN + n] * acc[k][ab];
+        });
+        if (f3) atomicAdd(&jpq[cd], cJ2 * jpq_s);
+        // K_MP[m][p] -= cMP sum_n D_nq I,  K_MQ[m][q] -= cMQ sum_n D_np I
+        static_for<0, NA>([&](auto m_) {
+            constexpr int m = decltype(m_)::value;
+            double smp = 0.0, smq = 0.0;
+            static_for<0, NB>([&](auto n_) {
+                constexpr int n = decltype(n_)::value;
+                const double x = acc[k][m * NB + n];
+                smp += D[(long)(oN + n) * ld + oQ + q] * x;
+                smq += D[(long)(oN + n) * ld + oP + p] * x;
+            });
+            atomicAdd(&strip[m * nbf + oP + p], -cMP * smp);
+            if (f2) atomicAdd(&strip[m * nbf + oQ + q], -cMQ * smq);
+        });
+        // K_NP[n][p] -= cNP sum_m D_mq I,  K_NQ[n][q] -= cNQ sum_m D_mp I
+        if (f1) {
+            static_for<0, NB>([&](auto n_) {
+                constexpr int n = decltype(n_)::value;
+                double snp = 0.0, snq = 0.0;
+                static_for<0, NA>([&](auto m_) {
+                    constexpr int m = decltype(m_)::value;
+                    const double x = acc[k][m * NB + n];
+                    snp += D[(long)(oM + m) * ld + oQ + q] * x;
+                    snq += D[(long)(oM + m) * ld + oP + p] * x;
+                });
+                atomicAdd(&strip[(NA + n) * nbf + oP + p], -cNP * snp);
+                if (f2) atomicAdd(&strip[(NA + n) * nbf + oQ + q], -cNQ * snq);
+            });
+        }
+    });
+}
+
+// The bra-row tile kernel for classes evaluated with bra and ket exchanged:
+// a thread owns NCT ket components and every bra component of its quartet,
+// so the Boys function and the R recursion of a primitive quartet are
+// computed by NCD / NCT threads instead of NAB / NC.  Units: (ket, ket
+// component chunk), no bra-primitive chunks.  LDS as eri_jk_tile_kernel.
+template <int LA, int LB, int LC, int LD, int NCT, typename Real>
+__global__ __launch_bounds__(256) void eri_jk_tile_t_kernel(EriArgs a)
+{
+    using S = EriShape<LA, LB, LC, LD>;
+    constexpr int NSPLIT = S::NCD / NCT;
+    constexpr int NA = S::NA, NB = S::NB, NCD = S::NCD, NAB = S::NAB, ND = S::ND;
+    constexpr bool LOW = std::is_same<Real, float>::value;
+    constexpr int L = LA + LB + LC + LD;
+    extern __shared__ double smem[];
+    __shared__ int s_task;
+    for (;;) {
+        if (threadIdx.x == 0) s_task = atomicAdd(a.counter, 1);
+        __syncthreads();
+        const int task = s_task;
+        if (task >= a.ntiles) break;
+        const int4 t = a.tiles[task];
+        const int bra = a.bra_list[t.x];
+        const int nbf = a.mld[a.pv.s1[bra]];
+        const int nstrip = (NA + NB) * nbf;
+        double* strip = smem;
+        double* jmn = smem + nstrip;
+        double* jpq = jmn + NA * NB;
+        const int nkt = t.z - t.y;
+        const int njpq = nkt * NCD;
+        for (int k = threadIdx.x; k < nstrip + NA * NB + njpq; k += 256) smem[k] = 0.0;
+        __syncthreads();
+        double jreg[NAB];
+        static_for<0, NAB>([&](auto c_) { jreg[decltype(c_)::value] = 0.0; });
+        const int nunits = nkt * NSPLIT;
+        for (int u = threadIdx.x; u < nunits; u += 256) {
+            const int q = u / NSPLIT;
+            const int cd0 = (u - q * NSPLIT) * NCT;
+            const int j = t.y + q;
+            if (!quartet_valid(a, t.x, j)) continue;
+            if ((a.vbra[t.x] * a.vket[j] >= a.tau2) == LOW) continue;
+            const int ket = a.ket_list[j];
+            double acc[NCT][NAB];
+            if constexpr (LOW) {
+                float accf[NCT][NAB];
+                eri_block<LC, LD, LA, LB, NCT, float>(a.pv, ket, bra, cd0, a.boys8f + L * BOYS8_SLAB, accf);
+                static_for<0, NCT>([&](auto c_) {
+                    static_for<0, NAB>([&](auto d_) {
+                        acc[decltype(c_)::value][decltype(d_)::value] =
+                            (double)accf[decltype(c_)::value][decltype(d_)::value];
+                    });
+                });
+            } else {
+                eri_block<LC, LD, LA, LB, NCT, double>(a.pv, ket, bra, cd0, a.boys8 + L * BOYS8_SLAB, acc);
+            }
+            digest_t<LA, LB, LC, LD, NCT>(a, bra, ket, cd0, acc, jreg, strip, nbf, jpq + q * NCD);
+        }
+        static_for<0, NAB>([&](auto c_) {
+            constexpr int c = decltype(c_)::value;
+            if (jreg[c] != 0.0) atomicAdd(&jmn[c], jreg[c]);
+        });
+        __syncthreads();
+        const int M = a.pv.s1[bra], N = a.pv.s2[bra];
+        const int oM = a.fstart[M], oN = a.fstart[N];
+        const long base = a.mbase[M];
+        const int ld = a.mld[M];
+        for (int k = threadIdx.x; k < nstrip; k += 256) {
+            const double v = strip[k];
+            if (v != 0.0) {
+                const int r = k / nbf, col = k - r * nbf;
+                const int row = r < NA ? oM + r : oN + (r - NA);
+                GTF_ATOMIC_ADD(&a.K[base + (long)row * ld + col], v);
+            }
+        }
+        for (int k = threadIdx.x; k < NA * NB; k += 256) {
+            const double v = jmn[k];
+            if (v != 0.0) GTF_ATOMIC_ADD(&a.J[base + (long)(oM + k / NB) * ld + oN + k % NB], v);
+        }
+        for (int k = threadIdx.x; k < njpq; k += 256) {
+            const double v = jpq[k];
+            if (v != 0.0) {
+                const int q = k / NCD, cd = k - q * NCD;
+                const int ket = a.ket_list[t.y + q];
+                const int oP = a.fstart[a.pv.s1[ket]], oQ = a.fstart[a.pv.s2[ket]];
+                GTF_ATOMIC_ADD(&a.J[base + (long)(oP + cd / ND) * ld + oQ + cd % ND], v);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // launch helpers (eri_table.cpp)
 typedef void (*eri_kernel_t)(EriArgs);
 struct EriClassInfo {
@@ -421,5 +578,9 @@ struct EriClassInfo {
     eri_kernel_t jk, diag, store;
     eri_kernel_t jk_tile;     // FP64 quartets (v_bra v_ket >= tau2)
     eri_kernel_t jk_tile_lo;  // FP32 quartets (v_bra v_ket < tau2)
+    // classes evaluated with bra and ket exchanged (nsplit_t > 0): threads
+    // per quartet and the two tile kernels
+    int nsplit_t;
+    eri_kernel_t jk_tile_t, jk_tile_t_lo;
 };
 const EriClassInfo* gtf_eri_class(int cls);

@@ -602,6 +602,13 @@ int gtf_pfock_set_timing(PFock_t pf, int on)
     return 0;
 }
 
+int gtf_pfock_set_serial(PFock_t pf, int on)
+{
+    if (!pf) return -1;
+    pf->sys.serial = on != 0;
+    return 0;
+}
+
 int gtf_pfock_class_times(PFock_t pf, double* ms, long* launches)
 {
     for (int c = 0; c < GTF_NCLASSES; c++) {

@@ -51,6 +51,10 @@ int gtf_pfock_mixed_counts(PFock_t pfock, long *nquartets_fp32, double *nprimqua
 /* Per-class kernel timing with HIP events recorded on the engine's stream
  * around each class launch; enabling resets the accumulators. */
 int gtf_pfock_set_timing(PFock_t pfock, int on);
+/* 1: launch the class kernels one after another on the engine's stream
+ * (each kernel alone on the GPU, for per-kernel measurement); 0 (default):
+ * concurrently on four streams. */
+int gtf_pfock_set_serial(PFock_t pfock, int on);
 /* Accumulated milliseconds and launch counts per class since enabling. */
 int gtf_pfock_class_times(PFock_t pfock, double *ms, long *launches);
 

@@ -153,3 +153,26 @@ def test_one_electron_matches_oracle(g, gbs, mol_dir):
     assert np.abs(X - oracle.inv_sqrt(ob.overlap())).max() < 1e-9
     assert np.abs(X @ S @ X - np.eye(b.nbf)).max() < 1e-10
     pf.close()
+
+
+@gpu
+@pytest.mark.parametrize("mask", ["0", "0xffffffff"])
+def test_fock_kernel_variants_match_oracle(g, gbs, mol_dir, monkeypatch, mask):
+    """Every class through the standard tile kernels (mask 0) and through the
+    bra/ket-exchanged ones wherever a class has one (all bits), FP64 and the
+    FP32 part of the mixed build."""
+    monkeypatch.setenv("GTF_JK_EXCHANGED", mask)
+    xyz = str(mol_dir / "butane.xyz")
+    b = g.BasisSet(gbs, xyz)
+    D = _rand_sym(b.nbf, 21)
+    pf = g.PFock(b, tolscr=1e-11)
+    F = pf.fock(D)
+    Jr, Kr, _ = oracle.Basis(gbs, xyz).fock(D)
+    err = np.abs(F - (Jr + Kr)).max()
+    pf.set_precision(1e-3)
+    F32 = pf.fock(D)
+    err32 = np.abs(F32 - F).max()
+    print("mask %s: max|dF| vs oracle %.2e; mixed (tau 1e-3) vs FP64 %.2e" % (mask, err, err32))
+    assert err < FOCK_TOL
+    assert err32 < 1e-6
+    pf.close()
