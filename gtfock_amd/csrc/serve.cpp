@@ -59,12 +59,18 @@ struct GtfServer {
     std::vector<SJob> jobs;
     double mem_in_use = 0, mem_peak = 0;
     int running = 0, ncohorts = 0, peak_running = 0;
+    // cohorts of large molecules running at once, and their cap: half the
+    // cohort slots, so smaller requests always find a slot
+    int running_large = 0, max_large = 1;
     bool stopping = false;
     std::thread scheduler;
     std::vector<std::thread> workers;
     void schedule();
-    void run_cohort(std::vector<long> ids, double mem, int cohort);
+    void run_cohort(std::vector<long> ids, double mem, int cohort, bool large);
 };
+
+// molecules of at least 384 basis functions count as large
+static const int LARGE_CLASS = 4;
 
 // power-of-two band of the basis size: 24..47 -> 0, 48..95 -> 1, ...
 static int size_class(int nbf)
@@ -92,7 +98,7 @@ static void publish(void* ctx, int m, double energy, int builds)
     c->s->cv.notify_all();
 }
 
-void GtfServer::run_cohort(std::vector<long> ids, double mem, int cohort)
+void GtfServer::run_cohort(std::vector<long> ids, double mem, int cohort, bool large)
 {
     std::vector<BasisSet_t> bases;
     {
@@ -126,6 +132,7 @@ void GtfServer::run_cohort(std::vector<long> ids, double mem, int cohort)
         }
     }
     running--;
+    if (large) running_large--;
     mem_in_use = running == 0 ? 0.0 : mem_in_use - mem;  // no rounding residue once idle
     cv.notify_all();
 }
@@ -157,8 +164,20 @@ void GtfServer::schedule()
             }
         }
         if (pending.empty()) continue;
-        // the oldest job opens the cohort; same-class jobs join, oldest first
-        const int cls = jobs[pending.front()].size_class;
+        // the oldest job opens the cohort (the oldest of a smaller class while
+        // the large-molecule slots are all taken); same-class jobs join,
+        // oldest first
+        int cls = -1;
+        for (long id : pending)
+            if (jobs[id].size_class < LARGE_CLASS || running_large < max_large) {
+                cls = jobs[id].size_class;
+                break;
+            }
+        if (cls < 0) {
+            // only large jobs are queued and their slots are taken
+            cv.wait(lk);
+            continue;
+        }
         std::vector<long> take;
         double mem = 0.0;
         for (auto itp = pending.begin(); itp != pending.end() && (int)take.size() < max_jobs;) {
@@ -185,9 +204,11 @@ void GtfServer::schedule()
         mem_in_use += mem;
         mem_peak = std::max(mem_peak, mem_in_use);
         running++;
+        const bool large = cls >= LARGE_CLASS;
+        if (large) running_large++;
         peak_running = std::max(peak_running, running);
         const int cohort = ncohorts++;
-        workers.emplace_back(&GtfServer::run_cohort, this, take, mem, cohort);
+        workers.emplace_back(&GtfServer::run_cohort, this, take, mem, cohort, large);
     }
 }
 
@@ -203,6 +224,7 @@ int gtf_server_create(double hbm_budget_bytes, int max_cohorts, int max_jobs_per
     if (!s) return -2;
     s->budget = hbm_budget_bytes;
     s->max_cohorts = max_cohorts;
+    s->max_large = std::max(1, max_cohorts / 2);
     s->max_jobs = max_jobs_per_cohort;
     s->tolscr = tolscr;
     s->tau = tau;

@@ -156,7 +156,9 @@ double gtf_batch_job_bytes(BasisSet_t basis);
  * oldest first, up to max_jobs_per_cohort and the device-memory budget
  * (gtf_batch_job_bytes per job).  Each cohort runs as one batch
  * (gtf_batch_scf_cb) on its own thread and streams, up to max_cohorts at
- * once; a job's result is published when its own SCF ends, its memory
+ * once, of which at most half hold molecules of 384 or more basis functions
+ * (the oldest job of a smaller class opens the next cohort while those slots
+ * are taken); a job's result is published when its own SCF ends, its memory
  * returns to the budget when its cohort ends.  The BasisSet of a job must
  * stay alive until the job is done. */
 typedef struct GtfServer *gtf_server_t;

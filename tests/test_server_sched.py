@@ -116,3 +116,22 @@ def test_destroy_cancels_queued_jobs(lib):
     time.sleep(0.05)
     L.gtf_server_destroy(h)  # waits for the running job, cancels the rest
     assert L.stub_created() == 1
+
+
+def test_large_molecules_take_at_most_half_the_slots(lib):
+    L = lib
+    L.stub_reset()
+    h = server(L, 1e13, 4, 1)  # one job per cohort
+    big = [L.stub_molecule(800, 0.4, -10.0) for _ in range(4)]
+    small = [L.stub_molecule(30, 0.05, -1.0) for _ in range(4)]
+    t0 = time.time()
+    ids_big = [L.gtf_server_submit(h, b) for b in big]
+    ids_small = [L.gtf_server_submit(h, b) for b in small]
+    L.gtf_server_wait(h)
+    done_small = max(poll(L, h, i)[1][4] for i in ids_small) - t0
+    done_big = sorted(poll(L, h, i)[1][4] - t0 for i in ids_big)
+    # two large cohorts at a time; the small jobs use the other two slots
+    assert done_small < 0.35
+    assert done_big[1] < 0.6 < done_big[2]
+    assert stats(L, h)["peak_running"] <= 4
+    L.gtf_server_destroy(h)
