@@ -272,6 +272,8 @@ def main():
     ap.add_argument("--serve-budget", type=float, default=64.0, help="HBM budget of the server, GiB")
     ap.add_argument("--serve-cohorts", type=int, default=4)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--scf-compare", type=int, default=0,
+                    help="SCF iterations of a per-iteration FP64 vs mixed energy comparison on the workload molecule")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -384,6 +386,21 @@ def main():
                  "dE_Eh": float(abs(np.sum(D * (F32 - F64))))}
         pf.set_precision(0.0)
 
+    scf_cmp = None
+    if args.scf_compare > 0:
+        log("%s: SCF FP64 vs mixed, %d iterations" % (args.workload, args.scf_compare))
+        b64 = g.Batch([basis])
+        e64, it64, h64 = b64.scf(max_iters=args.scf_compare, ediff=0.0, history=True)
+        b64.close()
+        b32 = g.Batch([basis], tau=args.tau)
+        e32, it32, h32 = b32.scf(max_iters=args.scf_compare, ediff=0.0, history=True)
+        b32.close()
+        d = np.abs(h32[0] - h64[0])
+        scf_cmp = {"iterations": args.scf_compare, "tau": args.tau, "E_fp64_final": float(h64[0][-1]),
+                   "dE_final_Eh": float(d[-1]), "dE_per_iteration_Eh": [float(x) for x in d],
+                   "dE_last_iterations_max_Eh": float(d[len(d) // 2:].max()),
+                   "E_change_last_iteration_fp64_Eh": float(abs(h64[0][-1] - h64[0][-2]))}
+
     served = None
     if args.serve > 0:
         log("serve leg: %d jobs at %.0f jobs/s" % (args.serve, args.serve_rate))
@@ -413,6 +430,7 @@ def main():
         "mixed": mixed,
         "batched": batched,
         "serve": served,
+        "scf_compare": scf_cmp,
         "cpu_baseline": cpu,
     }
     pf.close()
