@@ -212,6 +212,7 @@ int GtfSystem::screen(double tolscr)
     if (const char* s = getenv("GTF_JK_TILES")) use_tiles = atoi(s) != 0;
     if (const char* s = getenv("GTF_JK_SERIAL")) serial = atoi(s) != 0;
     if (const char* s = getenv("GTF_JK_EXCHANGED")) exchanged_mask = (unsigned)strtoul(s, nullptr, 0);
+    if (const char* s = getenv("GTF_JK_GRID")) tile_grid = std::max(1, atoi(s));
     {
         int dev = 0, lds = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
@@ -662,14 +663,14 @@ int GtfSystem::jk(const double* D, double* J, double* K)
                 a.tiles = d_tiles[cls];
                 a.ntiles = ntiles[cls];
                 a.counter = d_counters + cls;
-                if (launch(exch ? ci->jk_tile_t : ci->jk_tile, a, std::min(a.ntiles, 2048), cs, tile_lds[cls]))
+                if (launch(exch ? ci->jk_tile_t : ci->jk_tile, a, std::min(a.ntiles, tile_grid), cs, tile_lds[cls]))
                     return -1;
             }
             if (ntiles_lo[cls] > 0) {
                 a.tiles = d_tiles_lo[cls];
                 a.ntiles = ntiles_lo[cls];
                 a.counter = d_counters + GTF_NCLASSES + cls;
-                if (launch(exch ? ci->jk_tile_t_lo : ci->jk_tile_lo, a, std::min(a.ntiles, 2048), cs,
+                if (launch(exch ? ci->jk_tile_t_lo : ci->jk_tile_lo, a, std::min(a.ntiles, tile_grid), cs,
                            tile_lds[cls]))
                     return -1;
             }
