@@ -65,6 +65,8 @@ struct GtfServer {
     bool stopping = false;
     std::thread scheduler;
     std::vector<std::thread> workers;
+    std::vector<int> finished;  // cohorts whose thread has ended, not yet joined
+    std::vector<int> worker_cohort;
     void schedule();
     void run_cohort(std::vector<long> ids, double mem, int cohort, bool large);
 };
@@ -134,6 +136,7 @@ void GtfServer::run_cohort(std::vector<long> ids, double mem, int cohort, bool l
     running--;
     if (large) running_large--;
     mem_in_use = running == 0 ? 0.0 : mem_in_use - mem;  // no rounding residue once idle
+    finished.push_back(cohort);
     cv.notify_all();
 }
 
@@ -208,7 +211,27 @@ void GtfServer::schedule()
         if (large) running_large++;
         peak_running = std::max(peak_running, running);
         const int cohort = ncohorts++;
+        // take out the threads of cohorts that have ended (each has already
+        // published its results and returned its memory) and join them with
+        // the lock released, so a long-running server holds no dead threads
+        std::vector<std::thread> ended;
+        for (int c : finished)
+            for (size_t w = 0; w < workers.size(); w++)
+                if (worker_cohort[w] == c) {
+                    ended.push_back(std::move(workers[w]));
+                    workers.erase(workers.begin() + w);
+                    worker_cohort.erase(worker_cohort.begin() + w);
+                    break;
+                }
+        finished.clear();
         workers.emplace_back(&GtfServer::run_cohort, this, take, mem, cohort, large);
+        worker_cohort.push_back(cohort);
+        if (!ended.empty()) {
+            lk.unlock();
+            for (std::thread& t : ended)
+                if (t.joinable()) t.join();
+            lk.lock();
+        }
     }
 }
 

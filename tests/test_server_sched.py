@@ -130,8 +130,41 @@ def test_large_molecules_take_at_most_half_the_slots(lib):
     L.gtf_server_wait(h)
     done_small = max(poll(L, h, i)[1][4] for i in ids_small) - t0
     done_big = sorted(poll(L, h, i)[1][4] - t0 for i in ids_big)
-    # two large cohorts at a time; the small jobs use the other two slots
-    assert done_small < 0.35
-    assert done_big[1] < 0.6 < done_big[2]
+    # two large cohorts at a time; the small jobs use the other two slots, so
+    # they are all done before the first large job ends, and the large jobs
+    # end in two waves one large SCF time (0.4 s) apart.  Ordering and gaps,
+    # not absolute times, so a busy host does not fail the test.
+    assert done_small < done_big[0]
+    assert done_big[1] - done_big[0] < 0.15
+    assert done_big[2] - done_big[1] > 0.25
     assert stats(L, h)["peak_running"] <= 4
+    L.gtf_server_destroy(h)
+
+
+def _vm_kib():
+    with open("/proc/self/status") as f:
+        for line in f:
+            if line.startswith("VmSize:"):
+                return int(line.split()[1])
+    return 0
+
+
+def test_ended_cohort_threads_are_reclaimed(lib):
+    # one cohort per job, one at a time: 64 cohort threads start and end over
+    # the server's life.  A thread that has ended but is never joined keeps
+    # its stack mapped (8 MiB by default), so reclaiming them keeps the
+    # process's address space flat.
+    L = lib
+    L.stub_reset()
+    h = server(L, 1e12, 1, 1)
+    mols = [L.stub_molecule(30, 0.001, -1.0) for _ in range(4)]
+    ids = [L.gtf_server_submit(h, mols[k % 4]) for k in range(8)]
+    L.gtf_server_wait(h)
+    base = _vm_kib()
+    ids += [L.gtf_server_submit(h, mols[k % 4]) for k in range(64)]
+    L.gtf_server_wait(h)
+    grown = _vm_kib() - base
+    assert all(poll(L, h, i)[0] == 2 for i in ids)
+    assert stats(L, h)["cohorts"] == 72
+    assert grown < 64 * 1024, grown  # far below 64 unjoined 8 MiB stacks
     L.gtf_server_destroy(h)
