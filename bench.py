@@ -5,7 +5,9 @@ functions), Schwarz tolerance 1e-11, plain FP64 GPU drop-in.  One step is one
 PFock_computeFock -- screened ERIs of every class with fused J/K digestion and
 the symmetrisation -- with the density already resident in HBM (put and
 committed before the timed region).  The density is the first SCF density
-X purify(X H X) X, computed on the GPU.
+X purify(X H X) X, computed on the GPU.  The tile shapes of the quartet
+classes come from the library's autotuner, run on this molecule and density
+before the timed region (--autotune-reps 0: the built-in defaults).
 
 Prints ONE JSON line: value = Fock builds per second, with the roofline of
 the dominant kernel (event-timed per class launch on the engine's stream) and
@@ -82,53 +84,93 @@ def cpu_baseline(gbs, xyz, D, tol):
             "sample": "one full FP64 Fock build of the same molecule and density (%.1f s)" % dt}
 
 
-def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters):
-    """Many molecules per build: the seeded alkane / water-cluster batch.
-    Molecule-builds per second of (a) one PFock per molecule, FP64, built one
-    after another, (b) the batch, FP64, one HIP graph per build, (c) the
-    batch with FP32 ERIs below the Schwarz bound tau.  The density of each
-    molecule is its first SCF density.  With scf_iters > 0 the batched SCF is
-    run to convergence in FP64 and in mixed precision and the largest
-    total-energy difference is reported."""
+def batch_bases(g, gbs, nmol, seed):
+    """BasisSets of the seeded alkane / water-cluster batch distribution."""
     from gtfock_amd import geometry
     tmp = tempfile.mkdtemp()
-    mols = geometry.batch_molecules(nmol, seed)
     bases = []
-    for k, (name, atoms) in enumerate(mols):
+    for k, (name, atoms) in enumerate(geometry.batch_molecules(nmol, seed)):
         p = os.path.join(tmp, "m%03d.xyz" % k)
         with open(p, "w") as fh:
             fh.write(geometry.to_xyz(atoms))
         bases.append(g.BasisSet(gbs, p))
+    return bases
+
+
+def tune_on_sample(g, gbs, nmol, seed, tau, reps):
+    """Tile shapes from the autotuner, run on a sample batch of nmol
+    molecules drawn from the batch distribution with another seed (tuning
+    the full batch would cost more than one of its SCF runs; the shapes
+    carry over between batches of one distribution).  FP64 shapes, and with
+    tau > 0 the shapes of the mixed form."""
+    bases = batch_bases(g, gbs, nmol, seed)
+    bt = g.Batch(bases)
+    bt.scf(max_iters=2, ediff=0.0)
+    out = {}
+    for key, t in (("fp64", 0.0), ("mixed", tau)):
+        if key == "mixed":
+            bt.set_precision(t)
+        t0 = time.perf_counter()
+        r = bt.autotune(reps=reps)
+        r["seconds"] = time.perf_counter() - t0
+        out[key] = (bt.tile_shapes(), r)
+    bt.close()
+    return out
+
+
+def time_batch(bt, steps):
+    bt.compute()
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bt.compute()
+    device_sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters, tune_nmol, tune_reps):
+    """Many molecules per build: the seeded alkane / water-cluster batch.
+    Molecule-builds per second of (a) one PFock per molecule, FP64, built one
+    after another, (b) the batch, FP64, one HIP graph per build, with the
+    default tile shapes and with the autotuner's, (c) the batch with FP32
+    ERIs below the Schwarz bound tau (autotuned shapes).  The density of
+    each molecule is its first SCF density.  With scf_iters > 0 the batched
+    SCF is run to convergence in FP64 and in mixed precision and the largest
+    total-energy difference is reported."""
+    bases = batch_bases(g, gbs, nmol, seed)
     nbf = [b.nbf for b in bases]
+    tuned = None
+    if tune_nmol > 0 and tune_reps > 0:
+        log("batched: autotuner on a %d-molecule sample" % tune_nmol)
+        tuned = tune_on_sample(g, gbs, tune_nmol, seed + 1, tau, tune_reps)
     t0 = time.perf_counter()
     bt = g.Batch(bases)
     t_create = time.perf_counter() - t0
     bt.scf(max_iters=2, ediff=0.0)  # leaves the first SCF density on the device
     Ds = bt.get_density()
-    # (b) batched FP64, graph replays
-    bt.compute()
-    device_sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        bt.compute()
-    device_sync()
-    t_b64 = (time.perf_counter() - t0) / steps
+    log("batched: timed builds")
+    # (b) batched FP64, graph replays, default then tuned shapes
+    t_b64_default = time_batch(bt, steps)
+    if tuned:
+        for cls, sh in enumerate(tuned["fp64"][0]):
+            bt.set_tile_shape(cls, sh)
+        t_b64 = time_batch(bt, steps)
+    else:
+        t_b64 = t_b64_default
     G64 = bt.get_fock()
     # (c) batched mixed
     bt.set_precision(tau)
-    bt.compute()
-    device_sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        bt.compute()
-    device_sync()
-    t_b32 = (time.perf_counter() - t0) / steps
+    if tuned:
+        for cls, sh in enumerate(tuned["mixed"][0]):
+            bt.set_tile_shape(cls, sh)
+    t_b32 = time_batch(bt, steps)
     G32 = bt.get_fock()
     st = bt.stats()
     dF = max(float(np.abs(a - b).max()) for a, b in zip(G32, G64))
     dE1 = max(float(abs(np.sum(D * (a - b)))) for D, a, b in zip(Ds, G32, G64))
     bt.close()
-    # (a) one PFock per molecule, FP64, sequential
+    # (a) one PFock per molecule, FP64, sequential, default shapes
+    log("batched: one PFock per molecule")
     t_single = 0.0
     for b, D in zip(bases, Ds):
         pf = g.PFock(b)
@@ -137,18 +179,23 @@ def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters):
         pf.compute_fock()
         device_sync()
         t0 = time.perf_counter()
-        for _ in range(max(1, steps // 4)):
+        for _ in range(2):
             pf.compute_fock()
         device_sync()
-        t_single += (time.perf_counter() - t0) / max(1, steps // 4)
+        t_single += (time.perf_counter() - t0) / 2
         pf.close()
     out = {"molecules": nmol, "seed": seed, "nbf_min": min(nbf), "nbf_max": max(nbf),
            "nbf_mean": float(np.mean(nbf)), "create_s": round(t_create, 3),
-           "unbatched_fp64": nmol / t_single, "batched_fp64": nmol / t_b64, "batched_mixed": nmol / t_b32,
+           "unbatched_fp64": nmol / t_single, "batched_fp64": nmol / t_b64,
+           "batched_fp64_default_shapes": nmol / t_b64_default, "batched_mixed": nmol / t_b32,
            "unit": "molecule Fock builds/s", "tau": tau,
            "mixed_vs_unbatched_fp64": t_single / t_b32, "batched_fp64_vs_unbatched": t_single / t_b64,
+           "mixed_vs_batched_fp64": t_b64 / t_b32,
            "fp32_quartet_fraction": st["nq_fp32"] / max(1.0, st["nquartets"]),
            "max_abs_dF": dF, "max_first_order_dE_Eh": dE1}
+    if tuned:
+        out["autotune"] = {"sample_molecules": tune_nmol, "sample_seed": seed + 1, "reps": tune_reps,
+                           "fp64": tuned["fp64"][1], "mixed": tuned["mixed"][1]}
     log("batched: builds timed; SCF energy check")
     if scf_iters > 0:
         # a molecule counts when its FP64 SCF converged (|dE| < 1e-8 Eh between
@@ -156,11 +203,17 @@ def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters):
         # density, trace-correcting purification, DIIS) does not converge for
         # every geometry, and those are reported, not compared
         b64 = g.Batch(bases)
+        if tuned:
+            for cls, sh in enumerate(tuned["fp64"][0]):
+                b64.set_tile_shape(cls, sh)
         t0 = time.perf_counter()
         e64, it64 = b64.scf(max_iters=scf_iters, ediff=1e-8)
         t_scf64 = time.perf_counter() - t0
         b64.close()
         b32 = g.Batch(bases, tau=tau)
+        if tuned:
+            for cls, sh in enumerate(tuned["mixed"][0]):
+                b32.set_tile_shape(cls, sh)
         t0 = time.perf_counter()
         e32, it32 = b32.scf(max_iters=scf_iters, ediff=1e-8)
         t_scf = time.perf_counter() - t0
@@ -263,7 +316,11 @@ def main():
     ap.add_argument("--tau", type=float, default=1e-4, help="Schwarz-bound threshold of the FP32 ERIs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--classes", action="store_true", help="print the per-class breakdown to stderr")
-    ap.add_argument("--batch", type=int, default=32, help="molecules of the batched leg (0: skip)")
+    ap.add_argument("--batch", type=int, default=256, help="molecules of the batched leg (0: skip)")
+    ap.add_argument("--batch-tune", type=int, default=32,
+                    help="molecules of the sample batch the tile autotuner runs on (0: default shapes)")
+    ap.add_argument("--autotune-reps", type=int, default=3,
+                    help="launches per autotuner candidate, best taken (0: default tile shapes)")
     ap.add_argument("--batch-seed", type=int, default=20261202)
     ap.add_argument("--batch-scf", type=int, default=30, help="SCF iterations of the batched energy check (0: skip)")
     ap.add_argument("--serve", type=int, default=16, help="jobs of the serving leg (0: skip)")
@@ -299,6 +356,15 @@ def main():
     pf.commit()
     for _ in range(args.warmup):
         pf.compute_fock()
+    tune = None
+    if args.autotune_reps > 0:
+        # tile shapes from the autotuner, on this molecule and density
+        log("%s: tile autotuner" % args.workload)
+        t0 = time.perf_counter()
+        tune = {"fp64": pf.autotune(reps=args.autotune_reps)}
+        tune["fp64"]["seconds"] = time.perf_counter() - t0
+        for _ in range(args.warmup):
+            pf.compute_fock()
     pf.set_timing(True)
     log("%s: %d timed FP64 builds" % (args.workload, args.steps))
     device_sync()
@@ -367,6 +433,10 @@ def main():
         F64 = pf.get_mat(g.MAT_F)
         pf.set_timing(False)
         pf.set_precision(args.tau)
+        if tune is not None:
+            t0 = time.perf_counter()
+            tune["mixed"] = pf.autotune(reps=args.autotune_reps)
+            tune["mixed"]["seconds"] = time.perf_counter() - t0
         log("%s: %d timed mixed-precision builds" % (args.workload, args.steps))
         for _ in range(args.warmup):
             pf.compute_fock()
@@ -410,7 +480,8 @@ def main():
     batched = None
     if args.batch > 0:
         log("batched leg: %d molecules" % args.batch)
-        batched = batch_leg(g, gbs, args.batch, args.batch_seed, args.tau, max(2, args.steps // 4), args.batch_scf)
+        batched = batch_leg(g, gbs, args.batch, args.batch_seed, args.tau, max(2, args.steps // 4), args.batch_scf,
+                            args.batch_tune, max(1, args.autotune_reps - 1) if args.autotune_reps > 0 else 0)
 
     cpu = None
     if not args.no_cpu:
@@ -427,6 +498,7 @@ def main():
                    "nshells": basis.nshells, "tolscr": args.tol, "quartets": int(st["usq"]),
                    "integrals": int(st["uitl"]), "create_s": round(t_create, 3)},
         "roofline": roofline,
+        "autotune": tune,
         "mixed": mixed,
         "batched": batched,
         "serve": served,

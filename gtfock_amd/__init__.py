@@ -15,6 +15,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libgtfock_hip.so")
 DATA_DIR = os.path.join(_HERE, "data")
+# fields of a quartet class's tile shape (gtf_pfock_tile_shape)
+SHAPE_KEYS = ("units", "unit_pq", "tile_pq", "grid", "exchanged")
 CSRC_DIR = os.path.join(_HERE, "csrc")
 
 PFOCK_STATUS = {0: "SUCCESS", 1: "INIT_FAILED", 2: "ALLOC_FAILED", 3: "INVALID_VALUE",
@@ -116,6 +118,12 @@ def lib():
         "gtf_pfock_mixed_counts": ([_vp, _lp, _dp], ctypes.c_int),
         "gtf_pfock_class_times": ([_vp, _dp, _lp], ctypes.c_int),
         "gtf_pfock_device_bytes": ([_vp], ctypes.c_double),
+        "gtf_pfock_autotune": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+        "gtf_pfock_tile_shape": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
+        "gtf_pfock_set_tile_shape": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
+        "gtf_batch_autotune": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+        "gtf_batch_tile_shape": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
+        "gtf_batch_set_tile_shape": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
         "gtf_erd_quartets": ([_vp, ctypes.c_int, _ip, _dp], ctypes.c_int),
         "gtf_dgemm_host": ([ctypes.c_int, _dp, _dp, _dp, ctypes.c_int], ctypes.c_int),
         "gtf_gemm_host": ([ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
@@ -323,12 +331,33 @@ class PFock:
 
     def class_kernel(self, cls):
         """Name of the kernel that runs quartet class cls (FP64 part)."""
-        info = np.zeros(8, dtype=np.int32)
+        info = np.zeros(10, dtype=np.int32)
         _check(lib().gtf_pfock_class_info(self.h, cls, info.ctypes.data_as(_ip)), "gtf_pfock_class_info")
         la, lb, lc, ld, nc = (int(x) for x in info[:5])
+        if info[6] > 0 and info[8]:
+            return "eri_jk_tile_t_kernel<%d, %d, %d, %d, %d, double>" % (la, lb, lc, ld, int(info[9]))
         if info[6] > 0:
             return "eri_jk_tile_kernel<%d, %d, %d, %d, %d, double>" % (la, lb, lc, ld, nc)
         return "eri_kernel<%d, %d, %d, %d, %d, 0>" % (la, lb, lc, ld, nc)
+
+    def autotune(self, reps=3):
+        """Tile autotuner on density 0 (gtf_pfock_autotune): returns
+        {"start_ms", "tuned_ms", "candidates"}; the tuned shapes stay in
+        effect for later builds."""
+        ms = np.zeros(3)
+        _check(lib().gtf_pfock_autotune(self.h, int(reps), _p(ms)), "gtf_pfock_autotune")
+        return {"start_ms": float(ms[0]), "tuned_ms": float(ms[1]), "candidates": int(ms[2])}
+
+    def tile_shape(self, cls):
+        """{units, unit_pq, tile_pq, grid, exchanged} of quartet class cls."""
+        sh = np.zeros(5, dtype=np.int32)
+        st = lib().gtf_pfock_tile_shape(self.h, cls, sh.ctypes.data_as(_ip))
+        _check(0 if st == 5 else st, "gtf_pfock_tile_shape")
+        return dict(zip(SHAPE_KEYS, (int(x) for x in sh)))
+
+    def set_tile_shape(self, cls, shape):
+        sh = np.array([shape[k] for k in SHAPE_KEYS], dtype=np.int32)
+        _check(lib().gtf_pfock_set_tile_shape(self.h, cls, sh.ctypes.data_as(_ip)), "gtf_pfock_set_tile_shape")
 
     def set_serial(self, on):
         """Class kernels one after another (per-kernel measurement)."""
@@ -484,6 +513,25 @@ class Batch:
     def set_purif_precision(self, fp32, fp32_until=1e-3):
         self._st(lib().gtf_batch_set_purif_precision(self.h, 1 if fp32 else 0, fp32_until),
                  "gtf_batch_set_purif_precision")
+
+    def autotune(self, reps=3):
+        """Tile autotuner on the batch's density (gtf_batch_autotune)."""
+        ms = np.zeros(3)
+        self._st(lib().gtf_batch_autotune(self.h, int(reps), _p(ms)), "gtf_batch_autotune")
+        return {"start_ms": float(ms[0]), "tuned_ms": float(ms[1]), "candidates": int(ms[2])}
+
+    def tile_shape(self, cls):
+        sh = np.zeros(5, dtype=np.int32)
+        st = lib().gtf_batch_tile_shape(self.h, cls, sh.ctypes.data_as(_ip))
+        self._st(0 if st == 5 else st, "gtf_batch_tile_shape")
+        return dict(zip(SHAPE_KEYS, (int(x) for x in sh)))
+
+    def tile_shapes(self):
+        return [self.tile_shape(c) for c in range(21)]
+
+    def set_tile_shape(self, cls, shape):
+        sh = np.array([shape[k] for k in SHAPE_KEYS], dtype=np.int32)
+        self._st(lib().gtf_batch_set_tile_shape(self.h, cls, sh.ctypes.data_as(_ip)), "gtf_batch_set_tile_shape")
 
     def pack(self, mats):
         out = np.zeros(self.total)

@@ -505,6 +505,41 @@ int gtf_batch_set_precision(gtf_batch_t b, double tau)
     return b->sys.build_tiles();
 }
 
+int gtf_batch_autotune(gtf_batch_t b, int reps, double* ms)
+{
+    if (!b || reps <= 0) return -3;
+    const size_t bytes = sizeof(double) * (size_t)b->total;
+    double *J = nullptr, *K = nullptr;
+    if (hipMalloc(&J, bytes) != hipSuccess) return -2;
+    int rc = -2;
+    if (hipMalloc(&K, bytes) == hipSuccess) {
+        rc = -1;
+        if (hipStreamSynchronize(b->stream) == hipSuccess) {
+            // the captured build holds the old tile arrays
+            b->invalidate_graph();
+            rc = b->sys.autotune(b->d_D, J, K, reps, ms) ? -1 : 0;
+        }
+        (void)hipStreamSynchronize(b->stream);
+        (void)hipFree(K);
+    }
+    (void)hipFree(J);
+    return rc;
+}
+
+int gtf_batch_tile_shape(gtf_batch_t b, int cls, int* shape)
+{
+    if (!b) return -3;
+    return b->sys.get_shape(cls, shape) < 0 ? -3 : 5;
+}
+
+int gtf_batch_set_tile_shape(gtf_batch_t b, int cls, const int* shape)
+{
+    if (!b) return -3;
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return -1;
+    b->invalidate_graph();
+    return b->sys.set_shape(cls, shape);
+}
+
 int gtf_batch_set_purif_precision(gtf_batch_t b, int fp32, double fp32_until)
 {
     if (!b || fp32_until <= 0.0) return -3;

@@ -212,7 +212,8 @@ int GtfSystem::screen(double tolscr)
     if (const char* s = getenv("GTF_JK_TILES")) use_tiles = atoi(s) != 0;
     if (const char* s = getenv("GTF_JK_SERIAL")) serial = atoi(s) != 0;
     if (const char* s = getenv("GTF_JK_EXCHANGED")) exchanged_mask = (unsigned)strtoul(s, nullptr, 0);
-    if (const char* s = getenv("GTF_JK_GRID")) tile_grid = std::max(1, atoi(s));
+    if (const char* s = getenv("GTF_JK_GRID"))
+        for (TileShape& t : shape) t.grid = std::max(1, atoi(s));
     {
         int dev = 0, lds = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
@@ -493,31 +494,44 @@ void GtfSystem::csr(std::vector<int>& shellptr, std::vector<int>& shellid, std::
     shellptr[nshells] = (int)shellid.size();
 }
 
-// Tiles of up to GTF_TILE_UNITS work units of one bra row; (bra row, first
-// ket, end ket) with ket positions absolute in the ket type list.  With
+int GtfSystem::build_tiles()
+{
+    for (int cls = 0; cls < GTF_NCLASSES; cls++)
+        if (build_class_tiles(cls)) return -1;
+    SYS_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+// Tiles of one class: up to shape.units work units of one bra row; (bra row,
+// first ket, end ket) with ket positions absolute in the ket type list.  With
 // tau > 0 each row splits at the first ket whose Schwarz bound
 // sqrt(v_bra v_ket) < tau: FP64 tiles cover the block-padded prefix, FP32
 // tiles start at the block holding the split; the kernels re-test the exact
 // predicates per quartet.
-int GtfSystem::build_tiles()
+int GtfSystem::build_class_tiles(int cls)
 {
     const double tau2 = tau * tau;
-    for (int cls = 0; cls < GTF_NCLASSES; cls++) {
+    {
         int A, B;
         gtf_class_types(cls, A, B);
         const EriClassInfo* ci = gtf_eri_class(cls);
         const int NA = 2 * ci->la + 1, NB = 2 * ci->lb + 1;
         const int NCD = (2 * ci->lc + 1) * (2 * ci->ld + 1);
-        // threads per quartet of the kernel that runs the class
+        const TileShape sh = shape[cls];
+        // threads per quartet of the kernel that runs the class, and the most
+        // kets a tile can hold (its J_PQ rows live in LDS)
         const int ns = exchanged(cls) ? ci->nsplit_t : ci->nsplit;
-        tile_lds[cls] = sizeof(double) * ((size_t)(NA + NB) * nbf + NA * NB + (size_t)tile_kets(ns) * NCD);
+        const int kmax = std::max(1, sh.units / ns);
+        tile_lds[cls] = sizeof(double) * ((size_t)(NA + NB) * nbf + NA * NB + (size_t)kmax * NCD);
         ntiles[cls] = ntiles_lo[cls] = 0;
         nq_lo[cls] = 0;
         nprimq_lo[cls] = 0.0;
+        // the previous tiles may still be read by a launch in flight
+        SYS_CHECK(hipStreamSynchronize(stream));
         dfree(d_tiles[cls]);
         dfree(d_tiles_lo[cls]);
         d_tiles[cls] = d_tiles_lo[cls] = nullptr;
-        if (nq[cls] == 0 || !use_tiles || tile_lds[cls] > max_lds) continue;
+        if (nq[cls] == 0 || !use_tiles || tile_lds[cls] > max_lds) return 0;
         const std::vector<double>& vs = h_sorted[B];
         const std::vector<double> kp = ket_prim_prefix(B);
         // Tiles of the kets [b, e) of one row of segment sg, cut by work units
@@ -531,11 +545,11 @@ int GtfSystem::build_tiles()
         const bool exch = exchanged(cls);
         std::vector<std::pair<double, int4>> hi, lo;
         auto cut = [&](std::vector<std::pair<double, int4>>& out, int r, int nb, int nkmax, long b, long e) {
-            // bra-primitive chunks per quartet: about GTF_UNIT_PQ primitive
+            // bra-primitive chunks per quartet: about unit_pq primitive
             // quartets per unit; a power of two not above nb when one thread
             // owns the whole bra (its lanes then reduce with shuffles); none
             // for the exchanged kernels
-            const int ich = std::max(1, std::min(nb, (GTF_UNIT_PQ + nkmax - 1) / nkmax));
+            const int ich = std::max(1, std::min(nb, (sh.unit_pq + nkmax - 1) / nkmax));
             int nic = exch ? 1 : (nb + ich - 1) / ich;
             if (ns == 1) {
                 int p2 = 1;
@@ -543,13 +557,13 @@ int GtfSystem::build_tiles()
                 nic = p2;
             }
             // keep one ket's units within a tile
-            while (nic * ns > GTF_TILE_UNITS && nic > 1) nic = ns == 1 ? nic / 2 : nic - 1;
+            while (nic * ns > sh.units && nic > 1) nic = ns == 1 ? nic / 2 : nic - 1;
             const long per_ket = (long)nic * ns;
-            const long max_kets = std::max(1L, std::min<long>(GTF_TILE_UNITS / per_ket, tile_kets(ns)));
+            const long max_kets = std::max(1L, std::min<long>(sh.units / per_ket, kmax));
             long j = b;
             while (j < e) {
-                // the most kets whose primitive quartets stay within GTF_TILE_PQ
-                const double limit = lp[j] + (double)GTF_TILE_PQ / nb;
+                // the most kets whose primitive quartets stay within tile_pq
+                const double limit = lp[j] + (double)sh.tile_pq / nb;
                 long je = std::upper_bound(lp.begin() + j + 1, lp.begin() + e + 1, limit) - lp.begin() - 1;
                 je = std::max(j + 1, std::min(je, std::min(e, j + max_kets)));
                 out.push_back({(double)nb * (lp[je] - lp[j]), make_int4(r, (int)j, (int)je, nic)});
@@ -600,14 +614,12 @@ int GtfSystem::build_tiles()
             if (!d_tiles_lo[cls] || upload(d_tiles_lo[cls], lv, stream)) return -1;
         }
     }
-    SYS_CHECK(hipStreamSynchronize(stream));
     return 0;
 }
 
 int GtfSystem::jk(const double* D, double* J, double* K)
 {
     if (gtf_fill_int(stream, d_counters, 2 * GTF_NCLASSES, 0)) return -1;
-    const PairView pv = pair_view();
     if (!cstream[0]) {
         for (int s = 0; s < NSTREAMS; s++) {
             SYS_CHECK(hipStreamCreate(&cstream[s]));
@@ -625,58 +637,9 @@ int GtfSystem::jk(const double* D, double* J, double* K)
     for (int oi = 0; oi < GTF_NCLASSES; oi++) {
         const int cls = order[oi];
         if (nq[cls] == 0) continue;
-        int A, B;
-        gtf_class_types(cls, A, B);
-        const EriClassInfo* ci = gtf_eri_class(cls);
         hipStream_t cs = serial ? stream : cstream[slot++ % NSTREAMS];
-        EriArgs a{};
-        a.nunits = nslots[cls] * ci->nsplit;
-        a.row_off = d_rowoff[cls];
-        a.vbra = d_sigv[A];
-        a.vket = d_sigv[B];
-        a.rank_bra = d_rank[A];
-        a.rank_ket = d_rank[B];
-        a.tol2 = tol2;
-        a.same = A == B;
-        a.nrows = nrows[cls];
-        a.bra_list = d_sig[A];
-        a.ket_list = d_sig[B];
-        a.counter = d_counters + cls;
-        a.ntasks = (int)((a.nunits + 255) / 256);
-        a.pv = pv;
-        a.fstart = d_fstart;
-        a.boys8 = d_boys8;
-    a.boys8f = d_boys8f;
-        a.D = D;
-        a.J = J;
-        a.K = K;
-        a.mbase = d_mbase;
-        a.mld = d_mld;
-        a.row_bra = d_row_bra[cls];
-        a.row_kstart = d_row_kstart[cls];
         if (timing) SYS_CHECK(hipEventRecord(tev[2 * cls], cs));
-        if (ntiles[cls] > 0 || ntiles_lo[cls] > 0) {
-            a.nbf = nbf;
-            a.tau2 = tau * tau;
-            const bool exch = exchanged(cls);
-            if (ntiles[cls] > 0) {
-                a.tiles = d_tiles[cls];
-                a.ntiles = ntiles[cls];
-                a.counter = d_counters + cls;
-                if (launch(exch ? ci->jk_tile_t : ci->jk_tile, a, std::min(a.ntiles, tile_grid), cs, tile_lds[cls]))
-                    return -1;
-            }
-            if (ntiles_lo[cls] > 0) {
-                a.tiles = d_tiles_lo[cls];
-                a.ntiles = ntiles_lo[cls];
-                a.counter = d_counters + GTF_NCLASSES + cls;
-                if (launch(exch ? ci->jk_tile_t_lo : ci->jk_tile_lo, a, std::min(a.ntiles, tile_grid), cs,
-                           tile_lds[cls]))
-                    return -1;
-            }
-        } else if (launch(ci->jk, a, std::min(a.ntasks, 2048), cs)) {
-            return -1;
-        }
+        if (launch_class(cls, cs, D, J, K)) return -1;
         if (timing) SYS_CHECK(hipEventRecord(tev[2 * cls + 1], cs));
     }
     for (int s = 0; s < NSTREAMS; s++) {
@@ -684,6 +647,227 @@ int GtfSystem::jk(const double* D, double* J, double* K)
         SYS_CHECK(hipStreamWaitEvent(stream, join_ev[s], 0));
     }
     return 0;
+}
+
+int GtfSystem::launch_class(int cls, hipStream_t cs, const double* D, double* J, double* K)
+{
+    int A, B;
+    gtf_class_types(cls, A, B);
+    const EriClassInfo* ci = gtf_eri_class(cls);
+    const PairView pv = pair_view();
+    EriArgs a{};
+    a.nunits = nslots[cls] * ci->nsplit;
+    a.row_off = d_rowoff[cls];
+    a.vbra = d_sigv[A];
+    a.vket = d_sigv[B];
+    a.rank_bra = d_rank[A];
+    a.rank_ket = d_rank[B];
+    a.tol2 = tol2;
+    a.same = A == B;
+    a.nrows = nrows[cls];
+    a.bra_list = d_sig[A];
+    a.ket_list = d_sig[B];
+    a.counter = d_counters + cls;
+    a.ntasks = (int)((a.nunits + 255) / 256);
+    a.pv = pv;
+    a.fstart = d_fstart;
+    a.boys8 = d_boys8;
+    a.boys8f = d_boys8f;
+    a.D = D;
+    a.J = J;
+    a.K = K;
+    a.mbase = d_mbase;
+    a.mld = d_mld;
+    a.row_bra = d_row_bra[cls];
+    a.row_kstart = d_row_kstart[cls];
+    if (ntiles[cls] > 0 || ntiles_lo[cls] > 0) {
+        a.nbf = nbf;
+        a.tau2 = tau * tau;
+        const bool exch = exchanged(cls);
+        const int grid = shape[cls].grid;
+        if (ntiles[cls] > 0) {
+            a.tiles = d_tiles[cls];
+            a.ntiles = ntiles[cls];
+            a.counter = d_counters + cls;
+            if (launch(exch ? ci->jk_tile_t : ci->jk_tile, a, std::min(a.ntiles, grid), cs, tile_lds[cls])) return -1;
+        }
+        if (ntiles_lo[cls] > 0) {
+            a.tiles = d_tiles_lo[cls];
+            a.ntiles = ntiles_lo[cls];
+            a.counter = d_counters + GTF_NCLASSES + cls;
+            if (launch(exch ? ci->jk_tile_t_lo : ci->jk_tile_lo, a, std::min(a.ntiles, grid), cs, tile_lds[cls]))
+                return -1;
+        }
+        return 0;
+    }
+    return launch(ci->jk, a, std::min(a.ntasks, 2048), cs);
+}
+
+// best-of-reps event time of `run` (which enqueues on this->stream)
+template <typename F>
+static int best_time(hipStream_t st, int reps, float* ms, F run)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess) return -1;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return -1;
+    }
+    float best = 3.0e38f;
+    int rc = 0;
+    for (int r = 0; r < reps; r++) {
+        float t = 0.f;
+        if (hipEventRecord(e0, st) != hipSuccess || run() || hipEventRecord(e1, st) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+            rc = -1;
+            break;
+        }
+        best = std::min(best, t);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = best;
+    return rc;
+}
+
+int GtfSystem::time_class(int cls, const double* D, double* J, double* K, int reps, float* ms)
+{
+    if (gtf_fill_int(stream, d_counters, 2 * GTF_NCLASSES, 0)) return -1;
+    return best_time(stream, reps, ms, [&]() {
+        return gtf_fill_int(stream, d_counters, 2 * GTF_NCLASSES, 0) || launch_class(cls, stream, D, J, K);
+    });
+}
+
+int GtfSystem::time_build(const double* D, double* J, double* K, int reps, float* ms)
+{
+    return best_time(stream, reps, ms, [&]() { return jk(D, J, K); });
+}
+
+int GtfSystem::autotune(const double* D, double* J, double* K, int reps, double* ms_out)
+{
+    reps = std::max(1, reps);
+    TileShape start[GTF_NCLASSES];
+    std::copy(shape, shape + GTF_NCLASSES, start);
+    const unsigned start_mask = exchanged_mask;
+    const bool start_serial = serial, start_timing = timing;
+    serial = false;
+    timing = false;
+    long ntimed = 0;
+    float t_start = 0.f, t_tuned = 0.f;
+    int order[GTF_NCLASSES];
+    for (int c = 0; c < GTF_NCLASSES; c++) order[c] = c;
+    std::sort(order, order + GTF_NCLASSES, [&](int x, int y) { return nprimq[x] > nprimq[y]; });
+    int rc = time_build(D, J, K, reps, &t_start);
+    for (int oi = 0; oi < GTF_NCLASSES && rc == 0; oi++) {
+        const int cls = order[oi];
+        if (nq[cls] == 0 || (ntiles[cls] == 0 && ntiles_lo[cls] == 0)) continue;
+        const EriClassInfo* ci = gtf_eri_class(cls);
+        float best = 0.f;
+        if ((rc = time_class(cls, D, J, K, reps, &best))) break;
+        ntimed++;
+        // one coordinate: every candidate built and timed; the incumbent
+        // stays unless a candidate is at least 2% faster (timing noise)
+        auto coordinate = [&](int n, auto apply) -> int {
+            const TileShape keep = shape[cls];
+            const unsigned keep_mask = exchanged_mask;
+            TileShape best_sh = keep;
+            unsigned best_mask = keep_mask;
+            for (int k = 0; k < n; k++) {
+                shape[cls] = keep;
+                exchanged_mask = keep_mask;
+                if (!apply(k)) continue;  // same as the incumbent
+                if (build_class_tiles(cls)) return -1;
+                float t = 0.f;
+                if (time_class(cls, D, J, K, reps, &t)) return -1;
+                ntimed++;
+                if (t < 0.98f * best) {
+                    best = t;
+                    best_sh = shape[cls];
+                    best_mask = exchanged_mask;
+                }
+            }
+            shape[cls] = best_sh;
+            exchanged_mask = best_mask;
+            // rebuild what was chosen (the last candidate built may be another)
+            return build_class_tiles(cls);
+        };
+        const unsigned bit = 1u << cls;
+        if (ci->nsplit_t > 0 &&
+            (rc = coordinate(1, [&](int) {
+                 exchanged_mask ^= bit;
+                 return true;
+             })))
+            break;
+        static const int units[] = {256, 512, 1024, 2048, 4096};
+        if ((rc = coordinate(5, [&](int k) {
+                 if (shape[cls].units == units[k]) return false;
+                 shape[cls].units = units[k];
+                 return true;
+             })))
+            break;
+        static const int upq[] = {16, 32, 64, 128, 256};
+        if ((rc = coordinate(5, [&](int k) {
+                 if (shape[cls].unit_pq == upq[k]) return false;
+                 shape[cls].unit_pq = upq[k];
+                 return true;
+             })))
+            break;
+        static const int tpq[] = {16384, 65536, 262144, 1048576};
+        if ((rc = coordinate(4, [&](int k) {
+                 if (shape[cls].tile_pq == tpq[k]) return false;
+                 shape[cls].tile_pq = tpq[k];
+                 return true;
+             })))
+            break;
+        static const int grids[] = {512, 1024, 2048, 4096, 8192};
+        if ((rc = coordinate(5, [&](int k) {
+                 if (shape[cls].grid == grids[k]) return false;
+                 shape[cls].grid = grids[k];
+                 return true;
+             })))
+            break;
+    }
+    // the classes share the device in a build: keep the tuned shapes only if
+    // the whole concurrent build is faster with them
+    if (rc == 0) rc = time_build(D, J, K, reps, &t_tuned);
+    if (rc != 0 || t_tuned >= t_start) {
+        std::copy(start, start + GTF_NCLASSES, shape);
+        exchanged_mask = start_mask;
+        if (build_tiles()) rc = -1;
+        if (rc == 0) t_tuned = t_start;
+    }
+    serial = start_serial;
+    timing = start_timing;
+    if (ms_out) {
+        ms_out[0] = t_start;
+        ms_out[1] = t_tuned;
+        ms_out[2] = (double)ntimed;
+    }
+    return rc;
+}
+
+int GtfSystem::get_shape(int cls, int* out) const
+{
+    if (cls < 0 || cls >= GTF_NCLASSES || !out) return -1;
+    out[0] = shape[cls].units;
+    out[1] = shape[cls].unit_pq;
+    out[2] = shape[cls].tile_pq;
+    out[3] = shape[cls].grid;
+    out[4] = exchanged(cls) ? 1 : 0;
+    return 5;
+}
+
+int GtfSystem::set_shape(int cls, const int* in)
+{
+    if (cls < 0 || cls >= GTF_NCLASSES || !in) return -3;
+    if (in[0] < 1 || in[1] < 1 || in[2] < 1 || in[3] < 1 || in[4] < 0 || in[4] > 1) return -3;
+    if (in[4] == 1 && gtf_eri_class(cls)->nsplit_t == 0) return -3;
+    shape[cls].units = in[0];
+    shape[cls].unit_pq = in[1];
+    shape[cls].tile_pq = in[2];
+    shape[cls].grid = in[3];
+    exchanged_mask = in[4] ? exchanged_mask | (1u << cls) : exchanged_mask & ~(1u << cls);
+    return build_class_tiles(cls);
 }
 
 int GtfSystem::symmetrize(double* A)

@@ -96,10 +96,21 @@ struct GtfSystem {
     size_t tile_lds[GTF_NCLASSES] = {0};
     size_t max_lds = 65536;
     bool use_tiles = true;
-    // workgroups of a tile launch; each takes tiles from the class counter
-    // until none are left.  2048 persistent workgroups measured faster than
-    // one workgroup per tile (C10H22 7.6 vs 7.9 ms, C54H18 404 vs 433 ms)
-    int tile_grid = 2048;
+    // Tile shape of each class (the autotuner's search space): at most
+    // `units` work units and about `tile_pq` primitive quartets per tile,
+    // about `unit_pq` primitive quartets per unit (how finely a quartet's bra
+    // primitives are chunked), and `grid` persistent workgroups per launch,
+    // each taking tiles from the class counter until none are left.  The
+    // defaults are the shapes measured best on C10H22 / C54H18 by hand
+    // (2048 persistent workgroups beat one workgroup per tile: C10H22 7.6 vs
+    // 7.9 ms, C54H18 404 vs 433 ms).
+    struct TileShape {
+        int units = GTF_TILE_UNITS;
+        int unit_pq = GTF_UNIT_PQ;
+        int tile_pq = GTF_TILE_PQ;
+        int grid = 2048;
+    };
+    TileShape shape[GTF_NCLASSES];
     // classes run by the bra/ket-exchanged tile kernel (bit = class index;
     // only classes that have one).  Default: the two classes where it was
     // measured faster on MI355X, (pp|ps) and (dp|ps); in the others the large
@@ -130,6 +141,7 @@ struct GtfSystem {
     // Schwarz screening (pfock/screening.c:15-180) and quartet compaction
     int screen(double tolscr);
     int build_tiles();
+    int build_class_tiles(int cls);
     // CSR of molecule 0 in the reference layout (host)
     void csr(std::vector<int>& shellptr, std::vector<int>& shellid, std::vector<int>& shellrid,
              std::vector<double>& shellvalue) const;
@@ -145,6 +157,25 @@ struct GtfSystem {
     // fused ERI + J/K digestion over all screened quartets of every class;
     // D, J, K are the packed blocks of all molecules
     int jk(const double* D, double* J, double* K);
+    // the launches of one class on stream cs (its counters already zero)
+    int launch_class(int cls, hipStream_t cs, const double* D, double* J, double* K);
+    // Autotuner: per class, in decreasing order of work, a coordinate search
+    // over the exchanged kernel, units, unit_pq, tile_pq and grid, each
+    // candidate timed with the class alone on the stream (best of reps);
+    // then the whole concurrent build with the tuned shapes against the
+    // shapes it started from, keeping the faster.  D is the density the
+    // timings use (the J / K flushes skip zeros, so it should be a real one);
+    // J and K are scratch of mat_total doubles each.  ms_out (optional):
+    // [build ms with the starting shapes, build ms tuned, candidates timed].
+    int autotune(const double* D, double* J, double* K, int reps, double* ms_out);
+    int time_class(int cls, const double* D, double* J, double* K, int reps, float* ms);
+    // a class's shape as {units, unit_pq, tile_pq, grid, exchanged}; set
+    // rebuilds the class's tiles (exchanged = 1 only where the class has an
+    // exchanged kernel); set returns -3 for an invalid shape, -1 on a HIP
+    // failure
+    int get_shape(int cls, int* out) const;
+    int set_shape(int cls, const int* in);
+    int time_build(const double* D, double* J, double* K, int reps, float* ms);
     int collect_times();
     // A <- (A + A^T)/2 per molecule block
     int symmetrize(double* A);

@@ -266,28 +266,27 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
     }
 }
 
-// Fused ERI + J/K digestion over bra-row tiles.  A tile is up to
-// GTF_TILE_UNITS work units of one bra row; the workgroup accumulates J_MN and
+// Fused ERI + J/K digestion over bra-row tiles.  A tile is a bounded number
+// of work units of one bra row; the workgroup accumulates J_MN and
 // the exchange blocks of the bra shells in LDS and flushes them with one HBM
 // atomic per touched element when the tile is done.  The J_PQ block of
 // every ket of the tile is also kept in LDS and flushed at the end, so no
 // HBM atomic (whose completion the next loads would wait for) sits in the
-// quartet loop.  Dynamic LDS: (NA + NB) * nbf + NA * NB + tile_kets * NCD
-// doubles, nbf = the largest molecule's size.
+// quartet loop.  Dynamic LDS: (NA + NB) * nbf + NA * NB + kmax * NCD
+// doubles, nbf = the largest molecule's size, kmax = the most kets a tile of
+// the class may hold (units per tile / NSPLIT).
 //
 // A tile is (bra row, first ket, end ket, nic): each of its quartets is split
 // into nic bra-primitive chunks (sizes differing by at most one) times NSPLIT
 // bra-component chunks, so
-// that a unit holds about GTF_UNIT_PQ primitive quartets however contracted
-// the pairs are, and a tile holds at most GTF_TILE_UNITS units and about
-// GTF_TILE_PQ primitive quartets.  Digestion is linear in the integrals, so
-// every unit digests its partial sum on its own.
+// that a unit holds about unit_pq primitive quartets however contracted
+// the pairs are, and a tile holds at most `units` units and about tile_pq
+// primitive quartets (per class, chosen on the host; the defaults below).
+// Digestion is linear in the integrals, so every unit digests its partial
+// sum on its own.
 #define GTF_TILE_UNITS 1024
 #define GTF_UNIT_PQ 64
 #define GTF_TILE_PQ 65536
-
-// kets per tile of a class with nsplit threads per quartet
-__host__ __device__ constexpr int tile_kets(int nsplit) { return GTF_TILE_UNITS / nsplit > 0 ? GTF_TILE_UNITS / nsplit : 1; }
 
 template <int LA, int LB, int LC, int LD, int NC, typename Real>
 __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)

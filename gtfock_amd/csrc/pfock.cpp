@@ -559,6 +559,10 @@ int gtf_pfock_class_info(PFock_t pf, int cls, int* info)
     info[5] = ci->nsplit;
     info[6] = pf->sys.ntiles[cls];
     info[7] = pf->sys.ntiles_lo[cls];
+    // the exchanged tile kernel runs the class: ket components per thread
+    const bool ex = pf->sys.exchanged(cls);
+    info[8] = ex ? 1 : 0;
+    info[9] = ex ? (2 * ci->lc + 1) * (2 * ci->ld + 1) / ci->nsplit_t : 0;
     return 0;
 }
 
@@ -619,5 +623,33 @@ int gtf_pfock_class_times(PFock_t pf, double* ms, long* launches)
 }
 
 double gtf_pfock_device_bytes(PFock_t pf) { return (double)pf->sys.dev_bytes; }
+
+int gtf_pfock_autotune(PFock_t pf, int reps, double* ms)
+{
+    if (!pf || reps <= 0) return -1;
+    // scratch J / K: the build's own accumulators keep their contents
+    const size_t bytes = sizeof(double) * (size_t)pf->sys.mat_total;
+    double *J = nullptr, *K = nullptr;
+    if (hipMalloc(&J, bytes) != hipSuccess) return -1;
+    int rc = -1;
+    if (hipMalloc(&K, bytes) == hipSuccess) {
+        if (hipStreamSynchronize(pf->stream) == hipSuccess) rc = pf->sys.autotune(pf->d_D[0], J, K, reps, ms);
+        (void)hipStreamSynchronize(pf->stream);
+        (void)hipFree(K);
+    }
+    (void)hipFree(J);
+    return rc;
+}
+
+int gtf_pfock_tile_shape(PFock_t pf, int cls, int* shape)
+{
+    return pf ? pf->sys.get_shape(cls, shape) : -1;
+}
+
+int gtf_pfock_set_tile_shape(PFock_t pf, int cls, const int* shape)
+{
+    if (!pf || hipStreamSynchronize(pf->stream) != hipSuccess) return -1;
+    return pf->sys.set_shape(cls, shape) ? -1 : 0;
+}
 
 }  // extern "C"

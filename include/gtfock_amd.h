@@ -58,10 +58,12 @@ int gtf_pfock_set_serial(PFock_t pfock, int on);
 /* Accumulated milliseconds and launch counts per class since enabling. */
 int gtf_pfock_class_times(PFock_t pfock, double *ms, long *launches);
 
-/* Kernel shape of quartet class cls: info[8] = {La, Lb, Lc, Ld, bra
- * components per thread, threads per quartet, FP64 bra-row tiles, FP32
- * bra-row tiles}; with no tiles the class runs eri_kernel<..., EPI_JK>,
- * else eri_jk_tile_kernel<La, Lb, Lc, Ld, NC, double / float>. */
+/* Kernel shape of quartet class cls: info[10] = {La, Lb, Lc, Ld, bra
+ * components per thread NC, threads per quartet, FP64 bra-row tiles, FP32
+ * bra-row tiles, exchanged, ket components per thread NCT}; with no tiles
+ * the class runs eri_kernel<..., EPI_JK>, else eri_jk_tile_kernel<La, Lb,
+ * Lc, Ld, NC, double / float>, or eri_jk_tile_t_kernel<La, Lb, Lc, Ld, NCT,
+ * double / float> when exchanged = 1. */
 int gtf_pfock_class_info(PFock_t pfock, int cls, int *info);
 
 /* The engine's HIP stream (hipStream_t). */
@@ -69,6 +71,22 @@ void *gtf_pfock_stream(PFock_t pfock);
 
 /* Device memory held by the engine, in bytes. */
 double gtf_pfock_device_bytes(PFock_t pfock);
+
+/* Tile autotuner.  The tile launches of a quartet class have a shape
+ * shape[5] = {units, unit_pq, tile_pq, grid, exchanged}: at most `units`
+ * work units and about `tile_pq` primitive quartets per tile, about
+ * `unit_pq` primitive quartets per work unit (how finely a quartet's bra
+ * primitives are split), `grid` persistent workgroups, and exchanged = 1 for
+ * the bra/ket-exchanged kernel (only for classes that have one).
+ * gtf_pfock_autotune times the classes, each alone, over a coordinate
+ * search of these (best of reps launches per candidate) with density 0 as
+ * last put, then keeps the tuned shapes only if the whole build is faster
+ * with them.  ms[3] (may be NULL) = {build ms with the shapes it started
+ * from, build ms after tuning, candidates timed}.  Results do not depend on
+ * the shapes beyond FP64 summation order.  Returns 0, or -1 on failure. */
+int gtf_pfock_autotune(PFock_t pfock, int reps, double *ms);
+int gtf_pfock_tile_shape(PFock_t pfock, int cls, int *shape);
+int gtf_pfock_set_tile_shape(PFock_t pfock, int cls, const int *shape);
 
 /* Integrals of nq shell quartets (4 shell ids each) in the reference layout
  * iM + dM*(iN + dN*(iP + dP*iQ)), concatenated in input order. */
@@ -112,6 +130,13 @@ int gtf_batch_set_precision(gtf_batch_t batch, double tau);
 /* mixed-precision purification inside gtf_batch_scf, per molecule, as
  * gtf_purif_set_precision */
 int gtf_batch_set_purif_precision(gtf_batch_t batch, int fp32, double fp32_until);
+/* The tile autotuner on the batch's packed density, as gtf_pfock_autotune
+ * (the next build re-captures its graph); shapes per class as
+ * gtf_pfock_tile_shape / gtf_pfock_set_tile_shape, so the shapes tuned on
+ * one batch can be applied to another. */
+int gtf_batch_autotune(gtf_batch_t batch, int reps, double *ms);
+int gtf_batch_tile_shape(gtf_batch_t batch, int cls, int *shape);
+int gtf_batch_set_tile_shape(gtf_batch_t batch, int cls, const int *shape);
 /* 1 (default): replay the build as a captured HIP graph; 0: eager launches */
 int gtf_batch_use_graph(gtf_batch_t batch, int on);
 int gtf_batch_put_density(gtf_batch_t batch, int m, const double *D);
