@@ -6,8 +6,10 @@ PFock_computeFock -- screened ERIs of every class with fused J/K digestion and
 the symmetrisation -- with the density already resident in HBM (put and
 committed before the timed region).  The density is the first SCF density
 X purify(X H X) X, computed on the GPU.  The tile shapes of the quartet
-classes come from the library's autotuner, run on this molecule and density
-before the timed region (--autotune-reps 0: the built-in defaults).
+classes are the autotuner's: by default from the library's tuning table for
+the workload (written by `bench.py --tune live --write-shapes DIR`), with
+--tune live from the autotuner run on this molecule and density before the
+timed region, with --tune off the built-in defaults.
 
 Prints ONE JSON line: value = Fock builds per second, with the roofline of
 the dominant kernel (event-timed per class launch on the engine's stream) and
@@ -97,6 +99,37 @@ def batch_bases(g, gbs, nmol, seed):
     return bases
 
 
+class TileShapes:
+    """Where each leg's tile shapes come from (--tune): "db" the library's
+    tuning table for the leg when there is one (written earlier by the
+    autotuner, gtfock_amd/data/tuning/<name>.json), else the autotuner run
+    now; "live" the autotuner run now; "off" the built-in defaults.  With
+    write_dir, shapes tuned now are also written there as tables."""
+
+    def __init__(self, mode, write_dir, command):
+        self.mode, self.write_dir, self.command = mode, write_dir, command
+
+    def get(self, name, live):
+        """(shapes or None, record); live() runs the autotuner and returns
+        (shapes, autotuner result, description of what it was tuned on)."""
+        from gtfock_amd import tuning
+        if self.mode == "off":
+            return None, {"source": "built-in defaults"}
+        if self.mode == "db":
+            shapes, rec = tuning.load(name)
+            if shapes is not None:
+                return shapes, {"source": "tuning table gtfock_amd/data/tuning/%s.json" % name,
+                                "tuned": rec["tuned"]}
+        t0 = time.perf_counter()
+        shapes, res, on = live()
+        res = dict(res, tuned_on=on)
+        res.setdefault("seconds", time.perf_counter() - t0)
+        if self.write_dir:
+            tuning.save(name, shapes, dict(res, command=self.command, date=time.strftime("%Y-%m-%d")),
+                        directory=self.write_dir)
+        return shapes, {"source": "autotuner run in this process", "result": res}
+
+
 def tune_on_sample(g, gbs, nmol, seed, tau, reps):
     """Tile shapes from the autotuner, run on a sample batch of nmol
     molecules drawn from the batch distribution with another seed (tuning
@@ -107,13 +140,14 @@ def tune_on_sample(g, gbs, nmol, seed, tau, reps):
     bt = g.Batch(bases)
     bt.scf(max_iters=2, ediff=0.0)
     out = {}
+    on = "%d-molecule sample of the batch distribution, seed %d" % (nmol, seed)
     for key, t in (("fp64", 0.0), ("mixed", tau)):
         if key == "mixed":
             bt.set_precision(t)
         t0 = time.perf_counter()
         r = bt.autotune(reps=reps)
         r["seconds"] = time.perf_counter() - t0
-        out[key] = (bt.tile_shapes(), r)
+        out[key] = (bt.tile_shapes(), r, on + (", tau %g" % t if t else ""))
     bt.close()
     return out
 
@@ -128,7 +162,7 @@ def time_batch(bt, steps):
     return (time.perf_counter() - t0) / steps
 
 
-def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters, tune_nmol, tune_reps):
+def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters, tune_nmol, tune_reps, tiles):
     """Many molecules per build: the seeded alkane / water-cluster batch.
     Molecule-builds per second of (a) one PFock per molecule, FP64, built one
     after another, (b) the batch, FP64, one HIP graph per build, with the
@@ -137,12 +171,23 @@ def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters, tune_nmol, tune_reps):
     each molecule is its first SCF density.  With scf_iters > 0 the batched
     SCF is run to convergence in FP64 and in mixed precision and the largest
     total-energy difference is reported."""
+    from gtfock_amd import geometry
     bases = batch_bases(g, gbs, nmol, seed)
     nbf = [b.nbf for b in bases]
+    sample = {}
+
+    def live(key):
+        if not sample:
+            log("batched: autotuner on a %d-molecule sample" % tune_nmol)
+            sample.update(tune_on_sample(g, gbs, tune_nmol, seed + 1, tau, tune_reps))
+        return sample[key]
+
     tuned = None
     if tune_nmol > 0 and tune_reps > 0:
-        log("batched: autotuner on a %d-molecule sample" % tune_nmol)
-        tuned = tune_on_sample(g, gbs, tune_nmol, seed + 1, tau, tune_reps)
+        tuned = {"fp64": tiles.get("batch", lambda: live("fp64")),
+                 "mixed": tiles.get("batch_mixed", lambda: live("mixed"))}
+        if tuned["fp64"][0] is None:
+            tuned = None
     t0 = time.perf_counter()
     bt = g.Batch(bases)
     t_create = time.perf_counter() - t0
@@ -152,8 +197,7 @@ def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters, tune_nmol, tune_reps):
     # (b) batched FP64, graph replays, default then tuned shapes
     t_b64_default = time_batch(bt, steps)
     if tuned:
-        for cls, sh in enumerate(tuned["fp64"][0]):
-            bt.set_tile_shape(cls, sh)
+        bt.set_tile_shapes(tuned["fp64"][0])
         t_b64 = time_batch(bt, steps)
     else:
         t_b64 = t_b64_default
@@ -161,8 +205,7 @@ def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters, tune_nmol, tune_reps):
     # (c) batched mixed
     bt.set_precision(tau)
     if tuned:
-        for cls, sh in enumerate(tuned["mixed"][0]):
-            bt.set_tile_shape(cls, sh)
+        bt.set_tile_shapes(tuned["mixed"][0])
     t_b32 = time_batch(bt, steps)
     G32 = bt.get_fock()
     st = bt.stats()
@@ -194,40 +237,45 @@ def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters, tune_nmol, tune_reps):
            "fp32_quartet_fraction": st["nq_fp32"] / max(1.0, st["nquartets"]),
            "max_abs_dF": dF, "max_first_order_dE_Eh": dE1}
     if tuned:
-        out["autotune"] = {"sample_molecules": tune_nmol, "sample_seed": seed + 1, "reps": tune_reps,
-                           "fp64": tuned["fp64"][1], "mixed": tuned["mixed"][1]}
+        out["tile_shapes"] = {"fp64": tuned["fp64"][1], "mixed": tuned["mixed"][1]}
     log("batched: builds timed; SCF energy check")
     if scf_iters > 0:
-        # a molecule counts when its FP64 SCF converged (|dE| < 1e-8 Eh between
+        # a molecule counts when its SCF converged (|dE| < 1e-8 Eh between
         # builds) within scf_iters builds; the reference algorithm (zero initial
         # density, trace-correcting purification, DIIS) does not converge for
         # every geometry, and those are reported, not compared
         b64 = g.Batch(bases)
         if tuned:
-            for cls, sh in enumerate(tuned["fp64"][0]):
-                b64.set_tile_shape(cls, sh)
+            b64.set_tile_shapes(tuned["fp64"][0])
         t0 = time.perf_counter()
         e64, it64 = b64.scf(max_iters=scf_iters, ediff=1e-8)
         t_scf64 = time.perf_counter() - t0
         b64.close()
         b32 = g.Batch(bases, tau=tau)
         if tuned:
-            for cls, sh in enumerate(tuned["mixed"][0]):
-                b32.set_tile_shape(cls, sh)
+            b32.set_tile_shapes(tuned["mixed"][0])
         t0 = time.perf_counter()
         e32, it32 = b32.scf(max_iters=scf_iters, ediff=1e-8)
         t_scf = time.perf_counter() - t0
         b32.close()
-        conv = (it64 > 0) & (it64 < scf_iters)
-        de = np.abs(e32 - e64)[conv]
-        out.update({"scf_converged": int(conv.sum()), "scf_not_converged": int((~conv).sum()),
-                    "scf_max_abs_dE_Eh": float(de.max()) if de.size else None,
-                    "scf_builds_mean": float(np.mean(it64[conv])) if conv.any() else None,
+        # energies are compared where both forms converged; a molecule that
+        # converged in FP64 only is listed, with both runs' build counts
+        conv64 = (it64 > 0) & (it64 < scf_iters)
+        conv32 = (it32 > 0) & (it32 < scf_iters)
+        both = conv64 & conv32
+        de = np.abs(e32 - e64)
+        names = [n for n, _ in geometry.batch_molecules(nmol, seed)]
+        only64 = [{"molecule": names[k], "nbf": nbf[k], "builds_fp64": int(it64[k]), "builds_mixed": int(it32[k]),
+                   "E_fp64": float(e64[k]), "E_mixed": float(e32[k])} for k in np.flatnonzero(conv64 & ~conv32)]
+        out.update({"scf_converged": int(both.sum()), "scf_not_converged_fp64": int((~conv64).sum()),
+                    "scf_converged_fp64_only": only64,
+                    "scf_max_abs_dE_Eh": float(de[both].max()) if both.any() else None,
+                    "scf_builds_mean": float(np.mean(it64[both])) if both.any() else None,
                     "scf_fp64_s": t_scf64, "scf_mixed_s": t_scf})
     return out
 
 
-def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max_iters, ref):
+def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max_iters, ref, shapes=None):
     """SCF requests with seeded Poisson arrivals (rate jobs/s) submitted in
     real time to the request server (mixed precision, HBM-budget admission,
     several cohorts on the device at once).  Time to converge = done time -
@@ -247,7 +295,7 @@ def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max
     rng = np.random.default_rng(seed + 1)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=njobs))
     srv = g.Server(hbm_budget_gb=budget_gb, max_cohorts=cohorts, max_jobs_per_cohort=64, tau=tau,
-                   max_iters=max_iters, ediff=1e-8)
+                   max_iters=max_iters, ediff=1e-8, shapes=shapes)
     t0 = time.time()
     ids = []
     for b, at in zip(bases, arrivals):
@@ -320,7 +368,12 @@ def main():
     ap.add_argument("--batch-tune", type=int, default=32,
                     help="molecules of the sample batch the tile autotuner runs on (0: default shapes)")
     ap.add_argument("--autotune-reps", type=int, default=3,
-                    help="launches per autotuner candidate, best taken (0: default tile shapes)")
+                    help="launches per autotuner candidate, best taken")
+    ap.add_argument("--tune", choices=["db", "live", "off"], default="db",
+                    help="tile shapes: the library's tuning tables (autotuner run now where a leg has none), "
+                         "the autotuner run now, or the built-in defaults")
+    ap.add_argument("--write-shapes", default=None,
+                    help="directory to write the shapes tuned in this run to, as tuning tables")
     ap.add_argument("--batch-seed", type=int, default=20261202)
     ap.add_argument("--batch-scf", type=int, default=30, help="SCF iterations of the batched energy check (0: skip)")
     ap.add_argument("--serve", type=int, default=16, help="jobs of the serving leg (0: skip)")
@@ -356,15 +409,20 @@ def main():
     pf.commit()
     for _ in range(args.warmup):
         pf.compute_fock()
-    tune = None
-    if args.autotune_reps > 0:
-        # tile shapes from the autotuner, on this molecule and density
-        log("%s: tile autotuner" % args.workload)
-        t0 = time.perf_counter()
-        tune = {"fp64": pf.autotune(reps=args.autotune_reps)}
-        tune["fp64"]["seconds"] = time.perf_counter() - t0
-        for _ in range(args.warmup):
-            pf.compute_fock()
+    # tile shapes of the autotuner, tuned on this molecule and density
+    tiles = TileShapes(args.tune, args.write_shapes, " ".join(["python", "bench.py"] + sys.argv[1:]))
+
+    def live_molecule(what):
+        log("%s: tile autotuner (%s)" % (args.workload, what))
+        r = pf.autotune(reps=args.autotune_reps)
+        return pf.tile_shapes(), r, "%s, first SCF density, %s" % (args.workload, what)
+
+    sh, tune = tiles.get(args.workload, lambda: live_molecule("FP64"))
+    tune = {"fp64": tune}
+    if sh is not None:
+        pf.set_tile_shapes(sh)
+    for _ in range(args.warmup):
+        pf.compute_fock()
     pf.set_timing(True)
     log("%s: %d timed FP64 builds" % (args.workload, args.steps))
     device_sync()
@@ -433,10 +491,9 @@ def main():
         F64 = pf.get_mat(g.MAT_F)
         pf.set_timing(False)
         pf.set_precision(args.tau)
-        if tune is not None:
-            t0 = time.perf_counter()
-            tune["mixed"] = pf.autotune(reps=args.autotune_reps)
-            tune["mixed"]["seconds"] = time.perf_counter() - t0
+        sh, tune["mixed"] = tiles.get(args.workload + "_mixed", lambda: live_molecule("tau %g" % args.tau))
+        if sh is not None:
+            pf.set_tile_shapes(sh)
         log("%s: %d timed mixed-precision builds" % (args.workload, args.steps))
         for _ in range(args.warmup):
             pf.compute_fock()
@@ -474,14 +531,18 @@ def main():
     served = None
     if args.serve > 0:
         log("serve leg: %d jobs at %.0f jobs/s" % (args.serve, args.serve_rate))
+        from gtfock_amd import tuning
+        # cohorts run with the mixed-form shapes tuned on the batch distribution
+        serve_shapes, _ = tuning.load("batch_mixed") if args.tune == "db" else (None, None)
         served = serve_leg(g, gbs, args.serve, args.serve_rate, 20261203, args.tau, args.serve_rings,
-                           args.serve_budget, args.serve_cohorts, 40, True)
+                           args.serve_budget, args.serve_cohorts, 40, True, serve_shapes)
+        served["tile_shapes"] = "tuning table batch_mixed" if serve_shapes else "built-in defaults"
 
     batched = None
     if args.batch > 0:
         log("batched leg: %d molecules" % args.batch)
         batched = batch_leg(g, gbs, args.batch, args.batch_seed, args.tau, max(2, args.steps // 4), args.batch_scf,
-                            args.batch_tune, max(1, args.autotune_reps - 1) if args.autotune_reps > 0 else 0)
+                            args.batch_tune, max(1, args.autotune_reps - 1), tiles)
 
     cpu = None
     if not args.no_cpu:
@@ -498,7 +559,7 @@ def main():
                    "nshells": basis.nshells, "tolscr": args.tol, "quartets": int(st["usq"]),
                    "integrals": int(st["uitl"]), "create_s": round(t_create, 3)},
         "roofline": roofline,
-        "autotune": tune,
+        "tile_shapes": tune,
         "mixed": mixed,
         "batched": batched,
         "serve": served,

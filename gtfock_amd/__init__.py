@@ -16,7 +16,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libgtfock_hip.so")
 DATA_DIR = os.path.join(_HERE, "data")
 # fields of a quartet class's tile shape (gtf_pfock_tile_shape)
-SHAPE_KEYS = ("units", "unit_pq", "tile_pq", "grid", "exchanged")
+from .tuning import SHAPE_KEYS, flat as _flat_shapes  # noqa: E402
 CSRC_DIR = os.path.join(_HERE, "csrc")
 
 PFOCK_STATUS = {0: "SUCCESS", 1: "INIT_FAILED", 2: "ALLOC_FAILED", 3: "INVALID_VALUE",
@@ -121,6 +121,9 @@ def lib():
         "gtf_pfock_autotune": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
         "gtf_pfock_tile_shape": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
         "gtf_pfock_set_tile_shape": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
+        "gtf_pfock_set_tile_shapes": ([_vp, _ip], ctypes.c_int),
+        "gtf_batch_set_tile_shapes": ([_vp, _ip], ctypes.c_int),
+        "gtf_server_set_tile_shapes": ([_vp, _ip], ctypes.c_int),
         "gtf_batch_autotune": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
         "gtf_batch_tile_shape": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
         "gtf_batch_set_tile_shape": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
@@ -359,6 +362,14 @@ class PFock:
         sh = np.array([shape[k] for k in SHAPE_KEYS], dtype=np.int32)
         _check(lib().gtf_pfock_set_tile_shape(self.h, cls, sh.ctypes.data_as(_ip)), "gtf_pfock_set_tile_shape")
 
+    def tile_shapes(self):
+        return [self.tile_shape(c) for c in range(21)]
+
+    def set_tile_shapes(self, shapes):
+        """All 21 class shapes at once (e.g. a table from gtfock_amd.tuning)."""
+        sh = np.array(_flat_shapes(shapes), dtype=np.int32)
+        _check(lib().gtf_pfock_set_tile_shapes(self.h, sh.ctypes.data_as(_ip)), "gtf_pfock_set_tile_shapes")
+
     def set_serial(self, on):
         """Class kernels one after another (per-kernel measurement)."""
         _check(lib().gtf_pfock_set_serial(self.h, 1 if on else 0), "gtf_pfock_set_serial")
@@ -533,6 +544,10 @@ class Batch:
         sh = np.array([shape[k] for k in SHAPE_KEYS], dtype=np.int32)
         self._st(lib().gtf_batch_set_tile_shape(self.h, cls, sh.ctypes.data_as(_ip)), "gtf_batch_set_tile_shape")
 
+    def set_tile_shapes(self, shapes):
+        sh = np.array(_flat_shapes(shapes), dtype=np.int32)
+        self._st(lib().gtf_batch_set_tile_shapes(self.h, sh.ctypes.data_as(_ip)), "gtf_batch_set_tile_shapes")
+
     def pack(self, mats):
         out = np.zeros(self.total)
         for m, A in enumerate(mats):
@@ -601,7 +616,7 @@ class Server:
     JOB_KEYS = ["energy", "iters", "t_submit", "t_start", "t_done", "cohort", "mem_est"]
 
     def __init__(self, hbm_budget_gb=64.0, max_cohorts=4, max_jobs_per_cohort=64, tolscr=1e-11, tau=0.0,
-                 max_iters=30, ediff=1e-8):
+                 max_iters=30, ediff=1e-8, shapes=None):
         self.h = _vp()
         st = lib().gtf_server_create(hbm_budget_gb * 2.0 ** 30, max_cohorts, max_jobs_per_cohort, tolscr, tau,
                                      max_iters, ediff, ctypes.byref(self.h))
@@ -609,6 +624,18 @@ class Server:
             self.h = None
             raise GtfError("gtf_server_create failed (%d)" % st)
         self._keep = {}
+        if shapes is not None:
+            self.set_tile_shapes(shapes)
+
+    def set_tile_shapes(self, shapes):
+        """Tile shapes of the cohorts started from now on (None: defaults)."""
+        if shapes is None:
+            st = lib().gtf_server_set_tile_shapes(self.h, None)
+        else:
+            sh = np.array(_flat_shapes(shapes), dtype=np.int32)
+            st = lib().gtf_server_set_tile_shapes(self.h, sh.ctypes.data_as(_ip))
+        if st != 0:
+            raise GtfError("gtf_server_set_tile_shapes failed (%d)" % st)
 
     def submit(self, basis):
         jid = int(lib().gtf_server_submit(self.h, basis.h))

@@ -532,6 +532,14 @@ int gtf_batch_tile_shape(gtf_batch_t b, int cls, int* shape)
     return b->sys.get_shape(cls, shape) < 0 ? -3 : 5;
 }
 
+int gtf_batch_set_tile_shapes(gtf_batch_t b, const int* shapes)
+{
+    if (!b) return -3;
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return -1;
+    b->invalidate_graph();
+    return b->sys.set_shapes(shapes);
+}
+
 int gtf_batch_set_tile_shape(gtf_batch_t b, int cls, const int* shape)
 {
     if (!b) return -3;

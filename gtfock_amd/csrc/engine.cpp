@@ -857,17 +857,35 @@ int GtfSystem::get_shape(int cls, int* out) const
     return 5;
 }
 
-int GtfSystem::set_shape(int cls, const int* in)
+static bool shape_valid(int cls, const int* in)
 {
-    if (cls < 0 || cls >= GTF_NCLASSES || !in) return -3;
-    if (in[0] < 1 || in[1] < 1 || in[2] < 1 || in[3] < 1 || in[4] < 0 || in[4] > 1) return -3;
-    if (in[4] == 1 && gtf_eri_class(cls)->nsplit_t == 0) return -3;
+    if (in[0] < 1 || in[1] < 1 || in[2] < 1 || in[3] < 1 || in[4] < 0 || in[4] > 1) return false;
+    return !(in[4] == 1 && gtf_eri_class(cls)->nsplit_t == 0);
+}
+
+void GtfSystem::put_shape(int cls, const int* in)
+{
     shape[cls].units = in[0];
     shape[cls].unit_pq = in[1];
     shape[cls].tile_pq = in[2];
     shape[cls].grid = in[3];
     exchanged_mask = in[4] ? exchanged_mask | (1u << cls) : exchanged_mask & ~(1u << cls);
+}
+
+int GtfSystem::set_shape(int cls, const int* in)
+{
+    if (cls < 0 || cls >= GTF_NCLASSES || !in || !shape_valid(cls, in)) return -3;
+    put_shape(cls, in);
     return build_class_tiles(cls);
+}
+
+int GtfSystem::set_shapes(const int* in)
+{
+    if (!in) return -3;
+    for (int c = 0; c < GTF_NCLASSES; c++)
+        if (!shape_valid(c, in + 5 * c)) return -3;
+    for (int c = 0; c < GTF_NCLASSES; c++) put_shape(c, in + 5 * c);
+    return build_tiles();
 }
 
 int GtfSystem::symmetrize(double* A)

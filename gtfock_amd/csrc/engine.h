@@ -175,6 +175,10 @@ struct GtfSystem {
     // failure
     int get_shape(int cls, int* out) const;
     int set_shape(int cls, const int* in);
+    // every class's shape (GTF_NCLASSES x 5, as get_shape), one rebuild;
+    // nothing changes unless all are valid
+    int set_shapes(const int* in);
+    void put_shape(int cls, const int* in);
     int time_build(const double* D, double* J, double* K, int reps, float* ms);
     int collect_times();
     // A <- (A + A^T)/2 per molecule block

@@ -646,6 +646,12 @@ int gtf_pfock_tile_shape(PFock_t pf, int cls, int* shape)
     return pf ? pf->sys.get_shape(cls, shape) : -1;
 }
 
+int gtf_pfock_set_tile_shapes(PFock_t pf, const int* shapes)
+{
+    if (!pf || hipStreamSynchronize(pf->stream) != hipSuccess) return -1;
+    return pf->sys.set_shapes(shapes) ? -1 : 0;
+}
+
 int gtf_pfock_set_tile_shape(PFock_t pf, int cls, const int* shape)
 {
     if (!pf || hipStreamSynchronize(pf->stream) != hipSuccess) return -1;

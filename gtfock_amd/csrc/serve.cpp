@@ -67,6 +67,7 @@ struct GtfServer {
     std::vector<std::thread> workers;
     std::vector<int> finished;  // cohorts whose thread has ended, not yet joined
     std::vector<int> worker_cohort;
+    std::vector<int> shapes;  // tile shapes of new cohorts (empty: defaults)
     void schedule();
     void run_cohort(std::vector<long> ids, double mem, int cohort, bool large);
 };
@@ -103,8 +104,10 @@ static void publish(void* ctx, int m, double energy, int builds)
 void GtfServer::run_cohort(std::vector<long> ids, double mem, int cohort, bool large)
 {
     std::vector<BasisSet_t> bases;
+    std::vector<int> sh;
     {
         std::lock_guard<std::mutex> lk(mu);
+        sh = shapes;
         const double t = swtime();
         for (long id : ids) {
             bases.push_back(jobs[id].basis);
@@ -121,6 +124,7 @@ void GtfServer::run_cohort(std::vector<long> ids, double mem, int cohort, bool l
     int rc = gtf_batch_create(bases.data(), nmol, tolscr, &b);
     // the mixed-precision form: FP32 ERIs below the Schwarz bound tau
     if (rc == 0 && tau > 0.0) rc = gtf_batch_set_precision(b, tau);
+    if (rc == 0 && !sh.empty()) rc = gtf_batch_set_tile_shapes(b, sh.data());
     if (rc == 0) rc = gtf_batch_scf_cb(b, max_iters, ediff, e.data(), it.data(), nullptr, publish, &ctx);
     gtf_batch_destroy(b);
     std::lock_guard<std::mutex> lk(mu);
@@ -255,6 +259,17 @@ int gtf_server_create(double hbm_budget_bytes, int max_cohorts, int max_jobs_per
     s->ediff = ediff;
     s->scheduler = std::thread(&GtfServer::schedule, s);
     *out = s;
+    return 0;
+}
+
+int gtf_server_set_tile_shapes(gtf_server_t s, const int* shapes)
+{
+    if (!s) return -3;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (shapes)
+        s->shapes.assign(shapes, shapes + 21 * 5);
+    else
+        s->shapes.clear();
     return 0;
 }
 

@@ -87,6 +87,9 @@ double gtf_pfock_device_bytes(PFock_t pfock);
 int gtf_pfock_autotune(PFock_t pfock, int reps, double *ms);
 int gtf_pfock_tile_shape(PFock_t pfock, int cls, int *shape);
 int gtf_pfock_set_tile_shape(PFock_t pfock, int cls, const int *shape);
+/* all 21 classes at once, shapes[21 * 5] in class order; nothing changes
+ * unless every shape is valid */
+int gtf_pfock_set_tile_shapes(PFock_t pfock, const int *shapes);
 
 /* Integrals of nq shell quartets (4 shell ids each) in the reference layout
  * iM + dM*(iN + dN*(iP + dP*iQ)), concatenated in input order. */
@@ -137,6 +140,7 @@ int gtf_batch_set_purif_precision(gtf_batch_t batch, int fp32, double fp32_until
 int gtf_batch_autotune(gtf_batch_t batch, int reps, double *ms);
 int gtf_batch_tile_shape(gtf_batch_t batch, int cls, int *shape);
 int gtf_batch_set_tile_shape(gtf_batch_t batch, int cls, const int *shape);
+int gtf_batch_set_tile_shapes(gtf_batch_t batch, const int *shapes);
 /* 1 (default): replay the build as a captured HIP graph; 0: eager launches */
 int gtf_batch_use_graph(gtf_batch_t batch, int on);
 int gtf_batch_put_density(gtf_batch_t batch, int m, const double *D);
@@ -190,6 +194,10 @@ typedef struct GtfServer *gtf_server_t;
 
 int gtf_server_create(double hbm_budget_bytes, int max_cohorts, int max_jobs_per_cohort, double tolscr, double tau,
                       int max_iters, double ediff, gtf_server_t *server);
+/* tile shapes (21 x 5, as gtf_batch_set_tile_shapes) for the cohorts that
+ * start after the call, e.g. the autotuner's on a sample of the request
+ * distribution; NULL: the built-in defaults.  Returns 0 or -3. */
+int gtf_server_set_tile_shapes(gtf_server_t server, const int *shapes);
 /* returns the job id (>= 0) or -3 */
 long gtf_server_submit(gtf_server_t server, BasisSet_t basis);
 /* status: 0 queued, 1 running, 2 done, < 0 failed (-2: larger than the

@@ -55,6 +55,14 @@ void gtf_batch_destroy(gtf_batch_t b)
 }
 
 int gtf_batch_set_precision(gtf_batch_t, double) { return 0; }
+
+static std::atomic<int> g_shapes_set{0}, g_shape0_units{0};
+int gtf_batch_set_tile_shapes(gtf_batch_t, const int* shapes)
+{
+    g_shapes_set++;
+    g_shape0_units = shapes[0];
+    return 0;
+}
 int gtf_batch_set_purif_precision(gtf_batch_t, int, double) { return 0; }
 
 int gtf_batch_scf_cb(gtf_batch_t b, int, double, double* energy, int* iters, double*, gtf_scf_done_fn done,
@@ -79,10 +87,14 @@ BasisSet* stub_molecule(int nbf, double seconds, double energy) { return new Bas
 void stub_free(BasisSet* b) { delete b; }
 int stub_max_alive() { return g_max_alive.load(); }
 int stub_created() { return g_created.load(); }
+int stub_shapes_set() { return g_shapes_set.load(); }
+int stub_shape0_units() { return g_shape0_units.load(); }
 void stub_reset()
 {
     g_max_alive = 0;
     g_created = 0;
+    g_shapes_set = 0;
+    g_shape0_units = 0;
     std::lock_guard<std::mutex> lk(g_log_mu);
     g_cohort_sizes.clear();
 }

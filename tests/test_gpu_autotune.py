@@ -8,6 +8,9 @@ order.  The extreme shapes below reach the corner cases of the cut: one ket
 per tile, tiles whose J_PQ strip no longer fits in LDS (the class then runs
 the row kernel), and one persistent workgroup draining every tile.
 """
+import glob
+import os
+
 import numpy as np
 import pytest
 
@@ -158,3 +161,21 @@ def test_batch_autotune_and_shape_transfer(g, gbs, mol_dir):
     for a, c in zip(G0, G2):
         assert np.abs(a - c).max() < ORDER_TOL
     bt2.close()
+
+
+@gpu
+def test_committed_tuning_tables_match_oracle(g, gbs, butane):
+    """Every shape table the library ships applies and keeps oracle parity."""
+    from gtfock_amd import tuning
+    xyz, D, Fr = butane
+    b = g.BasisSet(gbs, xyz)
+    pf = g.PFock(b, tolscr=1e-11)
+    names = sorted(os.path.basename(p)[:-5] for p in glob.glob(os.path.join(tuning.TUNING_DIR, "*.json")))
+    assert names
+    for name in names:
+        shapes, _ = tuning.load(name)
+        print("table %s, case butane seed 31" % name, flush=True)
+        pf.set_tile_shapes(shapes)
+        assert pf.tile_shapes() == shapes
+        assert np.abs(pf.fock(D) - Fr).max() < FOCK_TOL
+    pf.close()

@@ -168,3 +168,25 @@ def test_ended_cohort_threads_are_reclaimed(lib):
     assert stats(L, h)["cohorts"] == 72
     assert grown < 64 * 1024, grown  # far below 64 unjoined 8 MiB stacks
     L.gtf_server_destroy(h)
+
+
+def test_tile_shapes_reach_the_cohorts_that_start_after(lib):
+    L = lib
+    L.stub_reset()
+    L.gtf_server_set_tile_shapes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+    h = server(L, 1e12, 1, 1)  # one job per cohort
+    mols = [L.stub_molecule(30, 0.01, -1.0) for _ in range(4)]
+    ids = [L.gtf_server_submit(h, mols[0])]
+    L.gtf_server_wait(h)
+    assert L.stub_shapes_set() == 0  # built-in defaults: nothing set
+    shapes = (ctypes.c_int * (21 * 5))(*([777, 64, 65536, 2048, 0] * 21))
+    assert L.gtf_server_set_tile_shapes(h, shapes) == 0
+    ids += [L.gtf_server_submit(h, m) for m in mols[1:3]]
+    L.gtf_server_wait(h)
+    assert L.stub_shapes_set() == 2 and L.stub_shape0_units() == 777
+    assert L.gtf_server_set_tile_shapes(h, None) == 0
+    ids.append(L.gtf_server_submit(h, mols[3]))
+    L.gtf_server_wait(h)
+    assert L.stub_shapes_set() == 2
+    assert all(poll(L, h, i)[0] == 2 for i in ids)
+    L.gtf_server_destroy(h)
