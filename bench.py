@@ -407,8 +407,6 @@ def main():
     D = first_density(g, pf, basis)
     pf.put_den_mat(D)
     pf.commit()
-    for _ in range(args.warmup):
-        pf.compute_fock()
     # tile shapes of the autotuner, tuned on this molecule and density
     tiles = TileShapes(args.tune, args.write_shapes, " ".join(["python", "bench.py"] + sys.argv[1:]))
 

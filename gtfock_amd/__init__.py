@@ -334,13 +334,14 @@ class PFock:
 
     def class_kernel(self, cls):
         """Name of the kernel that runs quartet class cls (FP64 part)."""
-        info = np.zeros(10, dtype=np.int32)
+        info = np.zeros(11, dtype=np.int32)
         _check(lib().gtf_pfock_class_info(self.h, cls, info.ctypes.data_as(_ip)), "gtf_pfock_class_info")
         la, lb, lc, ld, nc = (int(x) for x in info[:5])
+        blds = "true" if info[10] else "false"
         if info[6] > 0 and info[8]:
-            return "eri_jk_tile_t_kernel<%d, %d, %d, %d, %d, double>" % (la, lb, lc, ld, int(info[9]))
+            return "eri_jk_tile_t_kernel<%d, %d, %d, %d, %d, double, %s>" % (la, lb, lc, ld, int(info[9]), blds)
         if info[6] > 0:
-            return "eri_jk_tile_kernel<%d, %d, %d, %d, %d, double>" % (la, lb, lc, ld, nc)
+            return "eri_jk_tile_kernel<%d, %d, %d, %d, %d, double, %s>" % (la, lb, lc, ld, nc, blds)
         return "eri_kernel<%d, %d, %d, %d, %d, 0>" % (la, lb, lc, ld, nc)
 
     def autotune(self, reps=3):
@@ -352,10 +353,10 @@ class PFock:
         return {"start_ms": float(ms[0]), "tuned_ms": float(ms[1]), "candidates": int(ms[2])}
 
     def tile_shape(self, cls):
-        """{units, unit_pq, tile_pq, grid, exchanged} of quartet class cls."""
-        sh = np.zeros(5, dtype=np.int32)
+        """{units, unit_pq, tile_pq, grid, exchanged, boys_lds} of quartet class cls."""
+        sh = np.zeros(len(SHAPE_KEYS), dtype=np.int32)
         st = lib().gtf_pfock_tile_shape(self.h, cls, sh.ctypes.data_as(_ip))
-        _check(0 if st == 5 else st, "gtf_pfock_tile_shape")
+        _check(0 if st == len(SHAPE_KEYS) else st, "gtf_pfock_tile_shape")
         return dict(zip(SHAPE_KEYS, (int(x) for x in sh)))
 
     def set_tile_shape(self, cls, shape):
@@ -532,9 +533,9 @@ class Batch:
         return {"start_ms": float(ms[0]), "tuned_ms": float(ms[1]), "candidates": int(ms[2])}
 
     def tile_shape(self, cls):
-        sh = np.zeros(5, dtype=np.int32)
+        sh = np.zeros(len(SHAPE_KEYS), dtype=np.int32)
         st = lib().gtf_batch_tile_shape(self.h, cls, sh.ctypes.data_as(_ip))
-        self._st(0 if st == 5 else st, "gtf_batch_tile_shape")
+        self._st(0 if st == len(SHAPE_KEYS) else st, "gtf_batch_tile_shape")
         return dict(zip(SHAPE_KEYS, (int(x) for x in sh)))
 
     def tile_shapes(self):

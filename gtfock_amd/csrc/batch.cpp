@@ -529,7 +529,7 @@ int gtf_batch_autotune(gtf_batch_t b, int reps, double* ms)
 int gtf_batch_tile_shape(gtf_batch_t b, int cls, int* shape)
 {
     if (!b) return -3;
-    return b->sys.get_shape(cls, shape) < 0 ? -3 : 5;
+    return b->sys.get_shape(cls, shape) < 0 ? -3 : GTF_TILE_SHAPE_FIELDS;
 }
 
 int gtf_batch_set_tile_shapes(gtf_batch_t b, const int* shapes)

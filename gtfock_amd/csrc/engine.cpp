@@ -522,7 +522,8 @@ int GtfSystem::build_class_tiles(int cls)
         // kets a tile can hold (its J_PQ rows live in LDS)
         const int ns = exchanged(cls) ? ci->nsplit_t : ci->nsplit;
         const int kmax = std::max(1, sh.units / ns);
-        tile_lds[cls] = sizeof(double) * ((size_t)(NA + NB) * nbf + NA * NB + (size_t)kmax * NCD);
+        tile_lds[cls] = sizeof(double) * ((sh.boys_lds ? GTF_BOYS_LDS : 0) + (size_t)(NA + NB) * nbf + NA * NB +
+                                          (size_t)kmax * NCD);
         ntiles[cls] = ntiles_lo[cls] = 0;
         nq_lo[cls] = 0;
         nprimq_lo[cls] = 0.0;
@@ -685,17 +686,20 @@ int GtfSystem::launch_class(int cls, hipStream_t cs, const double* D, double* J,
         a.tau2 = tau * tau;
         const bool exch = exchanged(cls);
         const int grid = shape[cls].grid;
+        const int bl = shape[cls].boys_lds ? 1 : 0;
         if (ntiles[cls] > 0) {
             a.tiles = d_tiles[cls];
             a.ntiles = ntiles[cls];
             a.counter = d_counters + cls;
-            if (launch(exch ? ci->jk_tile_t : ci->jk_tile, a, std::min(a.ntiles, grid), cs, tile_lds[cls])) return -1;
+            if (launch(exch ? ci->jk_tile_t[bl] : ci->jk_tile[bl], a, std::min(a.ntiles, grid), cs, tile_lds[cls]))
+                return -1;
         }
         if (ntiles_lo[cls] > 0) {
             a.tiles = d_tiles_lo[cls];
             a.ntiles = ntiles_lo[cls];
             a.counter = d_counters + GTF_NCLASSES + cls;
-            if (launch(exch ? ci->jk_tile_t_lo : ci->jk_tile_lo, a, std::min(a.ntiles, grid), cs, tile_lds[cls]))
+            if (launch(exch ? ci->jk_tile_t_lo[bl] : ci->jk_tile_lo[bl], a, std::min(a.ntiles, grid), cs,
+                       tile_lds[cls]))
                 return -1;
         }
         return 0;
@@ -798,6 +802,11 @@ int GtfSystem::autotune(const double* D, double* J, double* K, int reps, double*
                  return true;
              })))
             break;
+        if ((rc = coordinate(1, [&](int) {
+                 shape[cls].boys_lds ^= 1;
+                 return true;
+             })))
+            break;
         static const int units[] = {256, 512, 1024, 2048, 4096};
         if ((rc = coordinate(5, [&](int k) {
                  if (shape[cls].units == units[k]) return false;
@@ -854,12 +863,14 @@ int GtfSystem::get_shape(int cls, int* out) const
     out[2] = shape[cls].tile_pq;
     out[3] = shape[cls].grid;
     out[4] = exchanged(cls) ? 1 : 0;
-    return 5;
+    out[5] = shape[cls].boys_lds;
+    return GTF_SHAPE_FIELDS;
 }
 
 static bool shape_valid(int cls, const int* in)
 {
-    if (in[0] < 1 || in[1] < 1 || in[2] < 1 || in[3] < 1 || in[4] < 0 || in[4] > 1) return false;
+    if (in[0] < 1 || in[1] < 1 || in[2] < 1 || in[3] < 1 || in[4] < 0 || in[4] > 1 || in[5] < 0 || in[5] > 1)
+        return false;
     return !(in[4] == 1 && gtf_eri_class(cls)->nsplit_t == 0);
 }
 
@@ -870,6 +881,7 @@ void GtfSystem::put_shape(int cls, const int* in)
     shape[cls].tile_pq = in[2];
     shape[cls].grid = in[3];
     exchanged_mask = in[4] ? exchanged_mask | (1u << cls) : exchanged_mask & ~(1u << cls);
+    shape[cls].boys_lds = in[5];
 }
 
 int GtfSystem::set_shape(int cls, const int* in)
@@ -883,8 +895,8 @@ int GtfSystem::set_shapes(const int* in)
 {
     if (!in) return -3;
     for (int c = 0; c < GTF_NCLASSES; c++)
-        if (!shape_valid(c, in + 5 * c)) return -3;
-    for (int c = 0; c < GTF_NCLASSES; c++) put_shape(c, in + 5 * c);
+        if (!shape_valid(c, in + GTF_SHAPE_FIELDS * c)) return -3;
+    for (int c = 0; c < GTF_NCLASSES; c++) put_shape(c, in + GTF_SHAPE_FIELDS * c);
     return build_tiles();
 }
 

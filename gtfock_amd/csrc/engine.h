@@ -17,6 +17,9 @@
 #include "eri_kernels.h"
 #include "gtf_internal.h"
 
+// ints per class in the tile-shape entry points
+#define GTF_SHAPE_FIELDS 6
+
 struct GtfSystem {
     hipStream_t stream = nullptr;
     // ---- molecules
@@ -99,8 +102,9 @@ struct GtfSystem {
     // Tile shape of each class (the autotuner's search space): at most
     // `units` work units and about `tile_pq` primitive quartets per tile,
     // about `unit_pq` primitive quartets per unit (how finely a quartet's bra
-    // primitives are chunked), and `grid` persistent workgroups per launch,
-    // each taking tiles from the class counter until none are left.  The
+    // primitives are chunked), `grid` persistent workgroups per launch, each
+    // taking tiles from the class counter until none are left, and
+    // `boys_lds` whether the workgroups stage the Boys slab in LDS.  The
     // defaults are the shapes measured best on C10H22 / C54H18 by hand
     // (2048 persistent workgroups beat one workgroup per tile: C10H22 7.6 vs
     // 7.9 ms, C54H18 404 vs 433 ms).
@@ -109,6 +113,7 @@ struct GtfSystem {
         int unit_pq = GTF_UNIT_PQ;
         int tile_pq = GTF_TILE_PQ;
         int grid = 2048;
+        int boys_lds = 0;
     };
     TileShape shape[GTF_NCLASSES];
     // classes run by the bra/ket-exchanged tile kernel (bit = class index;
@@ -169,13 +174,14 @@ struct GtfSystem {
     // [build ms with the starting shapes, build ms tuned, candidates timed].
     int autotune(const double* D, double* J, double* K, int reps, double* ms_out);
     int time_class(int cls, const double* D, double* J, double* K, int reps, float* ms);
-    // a class's shape as {units, unit_pq, tile_pq, grid, exchanged}; set
+    // a class's shape as {units, unit_pq, tile_pq, grid, exchanged,
+    // boys_lds} (GTF_SHAPE_FIELDS ints); set
     // rebuilds the class's tiles (exchanged = 1 only where the class has an
     // exchanged kernel); set returns -3 for an invalid shape, -1 on a HIP
     // failure
     int get_shape(int cls, int* out) const;
     int set_shape(int cls, const int* in);
-    // every class's shape (GTF_NCLASSES x 5, as get_shape), one rebuild;
+    // every class's shape (GTF_NCLASSES x GTF_SHAPE_FIELDS), one rebuild;
     // nothing changes unless all are valid
     int set_shapes(const int* in);
     void put_shape(int cls, const int* in);

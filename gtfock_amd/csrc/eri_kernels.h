@@ -272,9 +272,9 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
 // atomic per touched element when the tile is done.  The J_PQ block of
 // every ket of the tile is also kept in LDS and flushed at the end, so no
 // HBM atomic (whose completion the next loads would wait for) sits in the
-// quartet loop.  Dynamic LDS: (NA + NB) * nbf + NA * NB + kmax * NCD
-// doubles, nbf = the largest molecule's size, kmax = the most kets a tile of
-// the class may hold (units per tile / NSPLIT).
+// quartet loop.  Dynamic LDS: GTF_BOYS_LDS (the Boys slab, BLDS only) +
+// (NA + NB) * nbf + NA * NB + kmax * NCD doubles, nbf = the largest molecule's size, kmax =
+// the most kets a tile of the class may hold (units per tile / NSPLIT).
 //
 // A tile is (bra row, first ket, end ket, nic): each of its quartets is split
 // into nic bra-primitive chunks (sizes differing by at most one) times NSPLIT
@@ -288,7 +288,53 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
 #define GTF_UNIT_PQ 64
 #define GTF_TILE_PQ 65536
 
-template <int LA, int LB, int LC, int LD, int NC, typename Real>
+// The Boys Taylor slab of order L (FP64 or FP32) copied from HBM into the
+// first GTF_BOYS_LDS doubles of dynamic LDS.  A persistent workgroup copies
+// it once; every primitive quartet of every tile it takes then reads its
+// Taylor row from LDS instead of a dependent global load.  The barrier that
+// follows each tile's clearing of its LDS work area orders the copy before
+// the first read.  It costs 23 KB of LDS per workgroup, which lowers the
+// occupancy of classes whose strips are already large, so it is a per-class
+// choice (the tile shape's boys_lds).
+#define GTF_BOYS_LDS BOYS8_SLAB
+template <int L, typename Real>
+__device__ __forceinline__ const Real* boys_to_lds(const EriArgs& a, double* smem)
+{
+    Real* dst = reinterpret_cast<Real*>(smem);
+    const Real* src;
+    if constexpr (std::is_same<Real, float>::value)
+        src = a.boys8f + L * BOYS8_SLAB;
+    else
+        src = a.boys8 + L * BOYS8_SLAB;
+    for (int k = threadIdx.x; k < BOYS8_SLAB; k += 256) dst[k] = src[k];
+    return dst;
+}
+
+// the same slab read in place from HBM (through the caches)
+template <int L, typename Real>
+__device__ __forceinline__ const Real* boys_global(const EriArgs& a)
+{
+    if constexpr (std::is_same<Real, float>::value)
+        return a.boys8f + L * BOYS8_SLAB;
+    else
+        return a.boys8 + L * BOYS8_SLAB;
+}
+
+// Boys slab of a tile kernel: in LDS (BLDS; the work area then starts after
+// it) or read from HBM (the work area starts at the LDS base)
+template <int L, typename Real, bool BLDS>
+__device__ __forceinline__ const Real* tile_boys(const EriArgs& a, double* smem, double*& work)
+{
+    if constexpr (BLDS) {
+        work = smem + GTF_BOYS_LDS;
+        return boys_to_lds<L, Real>(a, smem);
+    } else {
+        work = smem;
+        return boys_global<L, Real>(a);
+    }
+}
+
+template <int LA, int LB, int LC, int LD, int NC, typename Real, bool BLDS>
 __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
 {
     using S = EriShape<LA, LB, LC, LD>;
@@ -300,6 +346,8 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
     constexpr int ND = S::ND;
     extern __shared__ double smem[];
     __shared__ int s_task;
+    double* work;
+    const Real* boys = tile_boys<L, Real, BLDS>(a, smem, work);
     for (;;) {
         if (threadIdx.x == 0) s_task = atomicAdd(a.counter, 1);
         __syncthreads();
@@ -312,11 +360,11 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
         // strip rows are as wide as the bra's molecule (at most a.nbf)
         const int nbf = a.mld[a.pv.s1[bra]];
         const int nstrip = (NA + NB) * nbf;
-        double* strip = smem;
-        double* jmn = smem + nstrip;
+        double* strip = work;
+        double* jmn = work + nstrip;
         double* jpq = jmn + NA * NB;
         const int njpq = JPQ_LDS ? (t.z - t.y) * NCD : 0;
-        for (int k = threadIdx.x; k < nstrip + NA * NB + njpq; k += 256) smem[k] = 0.0;
+        for (int k = threadIdx.x; k < nstrip + NA * NB + njpq; k += 256) work[k] = 0.0;
         __syncthreads();
         // Unit order.  NSPLIT == 1: the nic chunks of one quartet sit in
         // neighbouring lanes (nic is a power of two, so a group never spans
@@ -349,14 +397,14 @@ __global__ __launch_bounds__(256) void eri_jk_tile_kernel(EriArgs a)
             double acc[NC][NCD];
             if constexpr (LOW) {
                 float accf[NC][NCD];
-                eri_block<LA, LB, LC, LD, NC, float>(a.pv, bra, ket, c0, a.boys8f + L * BOYS8_SLAB, accf, i0, i1);
+                eri_block<LA, LB, LC, LD, NC, float>(a.pv, bra, ket, c0, boys, accf, i0, i1);
                 static_for<0, NC>([&](auto c_) {
                     static_for<0, NCD>([&](auto d_) {
                         acc[decltype(c_)::value][decltype(d_)::value] = (double)accf[decltype(c_)::value][decltype(d_)::value];
                     });
                 });
             } else {
-                eri_block<LA, LB, LC, LD, NC, double>(a.pv, bra, ket, c0, a.boys8 + L * BOYS8_SLAB, acc, i0, i1);
+                eri_block<LA, LB, LC, LD, NC, double>(a.pv, bra, ket, c0, boys, acc, i0, i1);
             }
             if constexpr (NSPLIT == 1) {
                 // sum the group's partial integrals into its first lane
@@ -482,7 +530,7 @@ __device__ __forceinline__ void digest_t(const EriArgs& a, int bra, int ket, int
 // so the Boys function and the R recursion of a primitive quartet are
 // computed by NCD / NCT threads instead of NAB / NC.  Units: (ket, ket
 // component chunk), no bra-primitive chunks.  LDS as eri_jk_tile_kernel.
-template <int LA, int LB, int LC, int LD, int NCT, typename Real>
+template <int LA, int LB, int LC, int LD, int NCT, typename Real, bool BLDS>
 __global__ __launch_bounds__(256) void eri_jk_tile_t_kernel(EriArgs a)
 {
     using S = EriShape<LA, LB, LC, LD>;
@@ -492,6 +540,8 @@ __global__ __launch_bounds__(256) void eri_jk_tile_t_kernel(EriArgs a)
     constexpr int L = LA + LB + LC + LD;
     extern __shared__ double smem[];
     __shared__ int s_task;
+    double* work;
+    const Real* boys = tile_boys<L, Real, BLDS>(a, smem, work);
     for (;;) {
         if (threadIdx.x == 0) s_task = atomicAdd(a.counter, 1);
         __syncthreads();
@@ -501,12 +551,12 @@ __global__ __launch_bounds__(256) void eri_jk_tile_t_kernel(EriArgs a)
         const int bra = a.bra_list[t.x];
         const int nbf = a.mld[a.pv.s1[bra]];
         const int nstrip = (NA + NB) * nbf;
-        double* strip = smem;
-        double* jmn = smem + nstrip;
+        double* strip = work;
+        double* jmn = work + nstrip;
         double* jpq = jmn + NA * NB;
         const int nkt = t.z - t.y;
         const int njpq = nkt * NCD;
-        for (int k = threadIdx.x; k < nstrip + NA * NB + njpq; k += 256) smem[k] = 0.0;
+        for (int k = threadIdx.x; k < nstrip + NA * NB + njpq; k += 256) work[k] = 0.0;
         __syncthreads();
         double jreg[NAB];
         static_for<0, NAB>([&](auto c_) { jreg[decltype(c_)::value] = 0.0; });
@@ -521,7 +571,7 @@ __global__ __launch_bounds__(256) void eri_jk_tile_t_kernel(EriArgs a)
             double acc[NCT][NAB];
             if constexpr (LOW) {
                 float accf[NCT][NAB];
-                eri_block<LC, LD, LA, LB, NCT, float>(a.pv, ket, bra, cd0, a.boys8f + L * BOYS8_SLAB, accf);
+                eri_block<LC, LD, LA, LB, NCT, float>(a.pv, ket, bra, cd0, boys, accf);
                 static_for<0, NCT>([&](auto c_) {
                     static_for<0, NAB>([&](auto d_) {
                         acc[decltype(c_)::value][decltype(d_)::value] =
@@ -529,7 +579,7 @@ __global__ __launch_bounds__(256) void eri_jk_tile_t_kernel(EriArgs a)
                     });
                 });
             } else {
-                eri_block<LC, LD, LA, LB, NCT, double>(a.pv, ket, bra, cd0, a.boys8 + L * BOYS8_SLAB, acc);
+                eri_block<LC, LD, LA, LB, NCT, double>(a.pv, ket, bra, cd0, boys, acc);
             }
             digest_t<LA, LB, LC, LD, NCT>(a, bra, ket, cd0, acc, jreg, strip, nbf, jpq + q * NCD);
         }
@@ -575,11 +625,12 @@ struct EriClassInfo {
     int nsplit;     // threads per quartet
     int nints;      // integrals per quartet
     eri_kernel_t jk, diag, store;
-    eri_kernel_t jk_tile;     // FP64 quartets (v_bra v_ket >= tau2)
-    eri_kernel_t jk_tile_lo;  // FP32 quartets (v_bra v_ket < tau2)
+    // tile kernels, [0] Boys slab read from HBM, [1] staged in LDS
+    eri_kernel_t jk_tile[2];     // FP64 quartets (v_bra v_ket >= tau2)
+    eri_kernel_t jk_tile_lo[2];  // FP32 quartets (v_bra v_ket < tau2)
     // classes evaluated with bra and ket exchanged (nsplit_t > 0): threads
     // per quartet and the two tile kernels
     int nsplit_t;
-    eri_kernel_t jk_tile_t, jk_tile_t_lo;
+    eri_kernel_t jk_tile_t[2], jk_tile_t_lo[2];
 };
 const EriClassInfo* gtf_eri_class(int cls);

@@ -563,6 +563,7 @@ int gtf_pfock_class_info(PFock_t pf, int cls, int* info)
     const bool ex = pf->sys.exchanged(cls);
     info[8] = ex ? 1 : 0;
     info[9] = ex ? (2 * ci->lc + 1) * (2 * ci->ld + 1) / ci->nsplit_t : 0;
+    info[10] = pf->sys.shape[cls].boys_lds;
     return 0;
 }
 
@@ -623,6 +624,8 @@ int gtf_pfock_class_times(PFock_t pf, double* ms, long* launches)
 }
 
 double gtf_pfock_device_bytes(PFock_t pf) { return (double)pf->sys.dev_bytes; }
+
+static_assert(GTF_SHAPE_FIELDS == GTF_TILE_SHAPE_FIELDS, "tile shape size");
 
 int gtf_pfock_autotune(PFock_t pf, int reps, double* ms)
 {

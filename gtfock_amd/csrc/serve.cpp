@@ -267,7 +267,7 @@ int gtf_server_set_tile_shapes(gtf_server_t s, const int* shapes)
     if (!s) return -3;
     std::lock_guard<std::mutex> lk(s->mu);
     if (shapes)
-        s->shapes.assign(shapes, shapes + 21 * 5);
+        s->shapes.assign(shapes, shapes + 21 * GTF_TILE_SHAPE_FIELDS);
     else
         s->shapes.clear();
     return 0;

@@ -58,12 +58,13 @@ int gtf_pfock_set_serial(PFock_t pfock, int on);
 /* Accumulated milliseconds and launch counts per class since enabling. */
 int gtf_pfock_class_times(PFock_t pfock, double *ms, long *launches);
 
-/* Kernel shape of quartet class cls: info[10] = {La, Lb, Lc, Ld, bra
+/* Kernel shape of quartet class cls: info[11] = {La, Lb, Lc, Ld, bra
  * components per thread NC, threads per quartet, FP64 bra-row tiles, FP32
- * bra-row tiles, exchanged, ket components per thread NCT}; with no tiles
- * the class runs eri_kernel<..., EPI_JK>, else eri_jk_tile_kernel<La, Lb,
- * Lc, Ld, NC, double / float>, or eri_jk_tile_t_kernel<La, Lb, Lc, Ld, NCT,
- * double / float> when exchanged = 1. */
+ * bra-row tiles, exchanged, ket components per thread NCT, boys_lds}; with
+ * no tiles the class runs eri_kernel<..., EPI_JK>, else
+ * eri_jk_tile_kernel<La, Lb, Lc, Ld, NC, double / float, boys_lds>, or
+ * eri_jk_tile_t_kernel<La, Lb, Lc, Ld, NCT, double / float, boys_lds> when
+ * exchanged = 1. */
 int gtf_pfock_class_info(PFock_t pfock, int cls, int *info);
 
 /* The engine's HIP stream (hipStream_t). */
@@ -73,12 +74,15 @@ void *gtf_pfock_stream(PFock_t pfock);
 double gtf_pfock_device_bytes(PFock_t pfock);
 
 /* Tile autotuner.  The tile launches of a quartet class have a shape
- * shape[5] = {units, unit_pq, tile_pq, grid, exchanged}: at most `units`
- * work units and about `tile_pq` primitive quartets per tile, about
- * `unit_pq` primitive quartets per work unit (how finely a quartet's bra
- * primitives are split), `grid` persistent workgroups, and exchanged = 1 for
- * the bra/ket-exchanged kernel (only for classes that have one).
- * gtf_pfock_autotune times the classes, each alone, over a coordinate
+ * shape[GTF_TILE_SHAPE_FIELDS] = {units, unit_pq, tile_pq, grid, exchanged,
+ * boys_lds}: at most `units` work units and about `tile_pq` primitive
+ * quartets per tile, about `unit_pq` primitive quartets per work unit (how
+ * finely a quartet's bra primitives are split), `grid` persistent
+ * workgroups, exchanged = 1 for the bra/ket-exchanged kernel (only for
+ * classes that have one), boys_lds = 1 for the kernels that stage the Boys
+ * table in LDS. */
+#define GTF_TILE_SHAPE_FIELDS 6
+/* gtf_pfock_autotune times the classes, each alone, over a coordinate
  * search of these (best of reps launches per candidate) with density 0 as
  * last put, then keeps the tuned shapes only if the whole build is faster
  * with them.  ms[3] (may be NULL) = {build ms with the shapes it started
@@ -87,8 +91,8 @@ double gtf_pfock_device_bytes(PFock_t pfock);
 int gtf_pfock_autotune(PFock_t pfock, int reps, double *ms);
 int gtf_pfock_tile_shape(PFock_t pfock, int cls, int *shape);
 int gtf_pfock_set_tile_shape(PFock_t pfock, int cls, const int *shape);
-/* all 21 classes at once, shapes[21 * 5] in class order; nothing changes
- * unless every shape is valid */
+/* all 21 classes at once, shapes[21 * GTF_TILE_SHAPE_FIELDS] in class
+ * order; nothing changes unless every shape is valid */
 int gtf_pfock_set_tile_shapes(PFock_t pfock, const int *shapes);
 
 /* Integrals of nq shell quartets (4 shell ids each) in the reference layout
@@ -194,7 +198,7 @@ typedef struct GtfServer *gtf_server_t;
 
 int gtf_server_create(double hbm_budget_bytes, int max_cohorts, int max_jobs_per_cohort, double tolscr, double tau,
                       int max_iters, double ediff, gtf_server_t *server);
-/* tile shapes (21 x 5, as gtf_batch_set_tile_shapes) for the cohorts that
+/* tile shapes (21 x GTF_TILE_SHAPE_FIELDS, as gtf_batch_set_tile_shapes) for the cohorts that
  * start after the call, e.g. the autotuner's on a sample of the request
  * distribution; NULL: the built-in defaults.  Returns 0 or -3. */
 int gtf_server_set_tile_shapes(gtf_server_t server, const int *shapes);

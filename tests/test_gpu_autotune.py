@@ -6,7 +6,8 @@ tiles and launched, so every shape must give the oracle's Fock matrix
 (1e-10 Eh max-abs) and the default shapes' matrix up to FP64 summation
 order.  The extreme shapes below reach the corner cases of the cut: one ket
 per tile, tiles whose J_PQ strip no longer fits in LDS (the class then runs
-the row kernel), and one persistent workgroup draining every tile.
+the row kernel), and one persistent workgroup draining every tile; each
+with the Boys table read from HBM and staged in LDS.
 """
 import glob
 import os
@@ -21,10 +22,12 @@ FOCK_TOL = 1e-10
 ORDER_TOL = 1e-11  # same quartets, another FP64 summation order
 
 EXTREME_SHAPES = {
-    "finest": dict(units=1, unit_pq=1, tile_pq=1, grid=2048),
-    "coarsest": dict(units=4096, unit_pq=256, tile_pq=1 << 22, grid=8192),
-    "one_workgroup": dict(units=1024, unit_pq=64, tile_pq=65536, grid=1),
-    "small": dict(units=256, unit_pq=16, tile_pq=16384, grid=512),
+    "finest": dict(units=1, unit_pq=1, tile_pq=1, grid=2048, boys_lds=0),
+    "finest_lds": dict(units=1, unit_pq=1, tile_pq=1, grid=2048, boys_lds=1),
+    "coarsest": dict(units=4096, unit_pq=256, tile_pq=1 << 22, grid=8192, boys_lds=0),
+    "coarsest_lds": dict(units=4096, unit_pq=256, tile_pq=1 << 22, grid=8192, boys_lds=1),
+    "one_workgroup_lds": dict(units=1024, unit_pq=64, tile_pq=65536, grid=1, boys_lds=1),
+    "small": dict(units=256, unit_pq=16, tile_pq=16384, grid=512, boys_lds=0),
 }
 
 
@@ -103,7 +106,7 @@ def test_invalid_tile_shapes_are_rejected(g, gbs, butane):
     b = g.BasisSet(gbs, xyz)
     pf = g.PFock(b, tolscr=1e-11)
     good = pf.tile_shape(0)
-    for bad in [dict(good, units=0), dict(good, grid=-1), dict(good, exchanged=2),
+    for bad in [dict(good, units=0), dict(good, grid=-1), dict(good, exchanged=2), dict(good, boys_lds=2),
                 dict(good, exchanged=1)]:  # class 0, (ss|ss), has no exchanged kernel
         with pytest.raises(g.GtfError):
             pf.set_tile_shape(0, bad)
