@@ -267,15 +267,42 @@ def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters, tune_nmol, tune_reps, t
         names = [n for n, _ in geometry.batch_molecules(nmol, seed)]
         only64 = [{"molecule": names[k], "nbf": nbf[k], "builds_fp64": int(it64[k]), "builds_mixed": int(it32[k]),
                    "E_fp64": float(e64[k]), "E_mixed": float(e32[k])} for k in np.flatnonzero(conv64 & ~conv32)]
+        # a molecule over 1e-6 Eh is run again, alone, in both forms: when its
+        # FP64 SCF does not reproduce itself either (several SCF solutions
+        # within reach of the FP64 summation-order noise), it says nothing
+        # about the mixed form and is listed apart
+        outliers, ill = [], set()
+        for k in np.flatnonzero(both & (de > 1e-6)):
+            rerun = {}
+            for key, t, sh in (("fp64", 0.0, tuned["fp64"][0] if tuned else None),
+                               ("mixed", tau, tuned["mixed"][0] if tuned else None)):
+                bk = g.Batch([bases[k]], tau=t)
+                if sh is not None:
+                    bk.set_tile_shapes(sh)
+                e, it = bk.scf(max_iters=scf_iters, ediff=1e-8)
+                bk.close()
+                rerun[key] = (float(e[0]), int(it[0]))
+            rec = {"molecule": names[k], "nbf": nbf[k], "E_fp64": float(e64[k]), "builds_fp64": int(it64[k]),
+                   "E_mixed": float(e32[k]), "builds_mixed": int(it32[k]),
+                   "E_fp64_rerun": rerun["fp64"][0], "builds_fp64_rerun": rerun["fp64"][1],
+                   "E_mixed_rerun": rerun["mixed"][0], "builds_mixed_rerun": rerun["mixed"][1]}
+            rec["fp64_reproducible"] = abs(rec["E_fp64_rerun"] - rec["E_fp64"]) < 1e-6
+            if not rec["fp64_reproducible"]:
+                ill.add(int(k))
+            outliers.append(rec)
+        ok = both.copy()
+        ok[list(ill)] = False
         out.update({"scf_converged": int(both.sum()), "scf_not_converged_fp64": int((~conv64).sum()),
                     "scf_converged_fp64_only": only64,
-                    "scf_max_abs_dE_Eh": float(de[both].max()) if both.any() else None,
+                    "scf_max_abs_dE_Eh": float(de[ok].max()) if ok.any() else None,
+                    "scf_compared": int(ok.sum()),
+                    "scf_over_1e-6": outliers,
                     "scf_builds_mean": float(np.mean(it64[both])) if both.any() else None,
                     "scf_fp64_s": t_scf64, "scf_mixed_s": t_scf})
     return out
 
 
-def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max_iters, ref, shapes=None):
+def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max_iters, ref, shapes=()):
     """SCF requests with seeded Poisson arrivals (rate jobs/s) submitted in
     real time to the request server (mixed precision, HBM-budget admission,
     several cohorts on the device at once).  Time to converge = done time -
@@ -295,7 +322,9 @@ def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max
     rng = np.random.default_rng(seed + 1)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=njobs))
     srv = g.Server(hbm_budget_gb=budget_gb, max_cohorts=cohorts, max_jobs_per_cohort=64, tau=tau,
-                   max_iters=max_iters, ediff=1e-8, shapes=shapes)
+                   max_iters=max_iters, ediff=1e-8)
+    for lo, hi, sh in shapes:
+        srv.set_tile_shapes_range(lo, hi, sh)
     t0 = time.time()
     ids = []
     for b, at in zip(bases, arrivals):
@@ -530,11 +559,19 @@ def main():
     if args.serve > 0:
         log("serve leg: %d jobs at %.0f jobs/s" % (args.serve, args.serve_rate))
         from gtfock_amd import tuning
-        # cohorts run with the mixed-form shapes tuned on the batch distribution
-        serve_shapes, _ = tuning.load("batch_mixed") if args.tune == "db" else (None, None)
+        # cohorts run with mixed-form shapes: those tuned on the batch
+        # distribution, and for flake cohorts (384 basis functions or more)
+        # those tuned on C54H18
+        ranges, used = [], []
+        if args.tune == "db":
+            for name, lo, hi in (("batch_mixed", 0, 1 << 30), ("c54h18_mixed", 384, 1 << 30)):
+                sh, _ = tuning.load(name)
+                if sh is not None:
+                    ranges.append((lo, hi, sh))
+                    used.append("%s for nbf %d.." % (name, lo))
         served = serve_leg(g, gbs, args.serve, args.serve_rate, 20261203, args.tau, args.serve_rings,
-                           args.serve_budget, args.serve_cohorts, 40, True, serve_shapes)
-        served["tile_shapes"] = "tuning table batch_mixed" if serve_shapes else "built-in defaults"
+                           args.serve_budget, args.serve_cohorts, 40, True, ranges)
+        served["tile_shapes"] = ("tuning tables: " + ", ".join(used)) if used else "built-in defaults"
 
     batched = None
     if args.batch > 0:

@@ -124,6 +124,7 @@ def lib():
         "gtf_pfock_set_tile_shapes": ([_vp, _ip], ctypes.c_int),
         "gtf_batch_set_tile_shapes": ([_vp, _ip], ctypes.c_int),
         "gtf_server_set_tile_shapes": ([_vp, _ip], ctypes.c_int),
+        "gtf_server_set_tile_shapes_range": ([_vp, ctypes.c_int, ctypes.c_int, _ip], ctypes.c_int),
         "gtf_batch_autotune": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
         "gtf_batch_tile_shape": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
         "gtf_batch_set_tile_shape": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
@@ -637,6 +638,14 @@ class Server:
             st = lib().gtf_server_set_tile_shapes(self.h, sh.ctypes.data_as(_ip))
         if st != 0:
             raise GtfError("gtf_server_set_tile_shapes failed (%d)" % st)
+
+    def set_tile_shapes_range(self, nbf_min, nbf_max, shapes):
+        """Shapes for cohorts whose largest molecule has nbf_min..nbf_max basis
+        functions (the last matching range wins)."""
+        sh = np.array(_flat_shapes(shapes), dtype=np.int32)
+        st = lib().gtf_server_set_tile_shapes_range(self.h, int(nbf_min), int(nbf_max), sh.ctypes.data_as(_ip))
+        if st != 0:
+            raise GtfError("gtf_server_set_tile_shapes_range failed (%d)" % st)
 
     def submit(self, basis):
         jid = int(lib().gtf_server_submit(self.h, basis.h))

@@ -21,6 +21,7 @@
 #include <sys/time.h>
 
 #include <algorithm>
+#include <climits>
 #include <condition_variable>
 #include <deque>
 #include <mutex>
@@ -45,6 +46,7 @@ struct SJob {
     int status = 0;  // 0 queued, 1 running, 2 done, < 0 failed
     int cohort = -1;
     int size_class = 0;
+    int nbf = 0;
     double mem_est = 0;
 };
 
@@ -67,7 +69,13 @@ struct GtfServer {
     std::vector<std::thread> workers;
     std::vector<int> finished;  // cohorts whose thread has ended, not yet joined
     std::vector<int> worker_cohort;
-    std::vector<int> shapes;  // tile shapes of new cohorts (empty: defaults)
+    // tile shapes of new cohorts by molecule size: a cohort takes the last
+    // range that holds its largest basis (none: the built-in defaults)
+    struct ShapeRange {
+        int nbf_min, nbf_max;
+        std::vector<int> shapes;
+    };
+    std::vector<ShapeRange> shape_ranges;
     void schedule();
     void run_cohort(std::vector<long> ids, double mem, int cohort, bool large);
 };
@@ -107,14 +115,17 @@ void GtfServer::run_cohort(std::vector<long> ids, double mem, int cohort, bool l
     std::vector<int> sh;
     {
         std::lock_guard<std::mutex> lk(mu);
-        sh = shapes;
         const double t = swtime();
+        int nbf_max = 0;
         for (long id : ids) {
             bases.push_back(jobs[id].basis);
             jobs[id].status = 1;
             jobs[id].t_start = t;
             jobs[id].cohort = cohort;
+            nbf_max = std::max(nbf_max, jobs[id].nbf);
         }
+        for (const ShapeRange& r : shape_ranges)
+            if (nbf_max >= r.nbf_min && nbf_max <= r.nbf_max) sh = r.shapes;
     }
     const int nmol = (int)ids.size();
     std::vector<double> e(nmol, 0.0);
@@ -266,10 +277,17 @@ int gtf_server_set_tile_shapes(gtf_server_t s, const int* shapes)
 {
     if (!s) return -3;
     std::lock_guard<std::mutex> lk(s->mu);
+    s->shape_ranges.clear();
     if (shapes)
-        s->shapes.assign(shapes, shapes + 21 * GTF_TILE_SHAPE_FIELDS);
-    else
-        s->shapes.clear();
+        s->shape_ranges.push_back({0, INT_MAX, std::vector<int>(shapes, shapes + 21 * GTF_TILE_SHAPE_FIELDS)});
+    return 0;
+}
+
+int gtf_server_set_tile_shapes_range(gtf_server_t s, int nbf_min, int nbf_max, const int* shapes)
+{
+    if (!s || !shapes || nbf_min < 0 || nbf_max < nbf_min) return -3;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->shape_ranges.push_back({nbf_min, nbf_max, std::vector<int>(shapes, shapes + 21 * GTF_TILE_SHAPE_FIELDS)});
     return 0;
 }
 
@@ -286,6 +304,7 @@ long gtf_server_submit(gtf_server_t s, BasisSet_t basis)
     j.t_submit = swtime();
     j.mem_est = bytes;
     j.size_class = size_class(nbf);
+    j.nbf = nbf;
     s->jobs.push_back(j);
     const long id = (long)s->jobs.size() - 1;
     s->pending.push_back(id);

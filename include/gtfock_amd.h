@@ -202,6 +202,11 @@ int gtf_server_create(double hbm_budget_bytes, int max_cohorts, int max_jobs_per
  * start after the call, e.g. the autotuner's on a sample of the request
  * distribution; NULL: the built-in defaults.  Returns 0 or -3. */
 int gtf_server_set_tile_shapes(gtf_server_t server, const int *shapes);
+/* shapes for the cohorts (started after the call) whose largest molecule has
+ * nbf_min..nbf_max basis functions, on top of the shapes set so far (the
+ * last matching range wins; gtf_server_set_tile_shapes clears the ranges).
+ * Returns 0 or -3. */
+int gtf_server_set_tile_shapes_range(gtf_server_t server, int nbf_min, int nbf_max, const int *shapes);
 /* returns the job id (>= 0) or -3 */
 long gtf_server_submit(gtf_server_t server, BasisSet_t basis);
 /* status: 0 queued, 1 running, 2 done, < 0 failed (-2: larger than the

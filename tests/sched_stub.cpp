@@ -57,10 +57,13 @@ void gtf_batch_destroy(gtf_batch_t b)
 int gtf_batch_set_precision(gtf_batch_t, double) { return 0; }
 
 static std::atomic<int> g_shapes_set{0}, g_shape0_units{0};
+static std::vector<int> g_shape_units;  // class-0 units of every set, in order
 int gtf_batch_set_tile_shapes(gtf_batch_t, const int* shapes)
 {
     g_shapes_set++;
     g_shape0_units = shapes[0];
+    std::lock_guard<std::mutex> lk(g_log_mu);
+    g_shape_units.push_back(shapes[0]);
     return 0;
 }
 int gtf_batch_set_purif_precision(gtf_batch_t, int, double) { return 0; }
@@ -89,12 +92,23 @@ int stub_max_alive() { return g_max_alive.load(); }
 int stub_created() { return g_created.load(); }
 int stub_shapes_set() { return g_shapes_set.load(); }
 int stub_shape0_units() { return g_shape0_units.load(); }
+int stub_shape_units(int* out, int cap)
+{
+    std::lock_guard<std::mutex> lk(g_log_mu);
+    const int n = (int)g_shape_units.size();
+    for (int i = 0; i < n && i < cap; i++) out[i] = g_shape_units[i];
+    return n;
+}
 void stub_reset()
 {
     g_max_alive = 0;
     g_created = 0;
     g_shapes_set = 0;
     g_shape0_units = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_log_mu);
+        g_shape_units.clear();
+    }
     std::lock_guard<std::mutex> lk(g_log_mu);
     g_cohort_sizes.clear();
 }

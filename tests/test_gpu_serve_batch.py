@@ -193,3 +193,28 @@ def test_server_budget_admission(g, gbs, small_batch):
     print([r["status"] for r in res], st)
     assert st["peak_bytes"] <= st["budget"]
     assert [r["status"] for r in res] == [2, 2, 2, -2, 2, 2, 2]
+
+
+@gpu
+def test_server_tuned_shapes_keep_energies(g, gbs, small_batch):
+    """Cohorts running with the shipped tuning tables (by molecule size, as
+    the serving benchmark sets them) give the energies of cohorts running
+    with the built-in shapes: a tile shape changes only summation order."""
+    from gtfock_amd import tuning
+    bases = [g.BasisSet(gbs, p) for p in small_batch]
+    small, _ = tuning.load("batch_mixed")
+    large, _ = tuning.load("c54h18_mixed")
+    res = []
+    for ranges in ([], [(0, 1 << 30, small), (60, 1 << 30, large)]):
+        s = g.Server(hbm_budget_gb=8.0, max_cohorts=2, max_jobs_per_cohort=2, tau=1e-4, max_iters=20, ediff=1e-10)
+        for lo, hi, sh in ranges:
+            s.set_tile_shapes_range(lo, hi, sh)
+        ids = [s.submit(b) for b in bases]
+        s.wait()
+        res.append([s.poll(i) for i in ids])
+        s.close()
+    for k, (a, b) in enumerate(zip(*res)):
+        print("job %d (nbf %d): default %.12f (%d), tuned %.12f (%d)" % (
+            k, bases[k].nbf, a["energy"], a["iters"], b["energy"], b["iters"]))
+        assert a["status"] == 2 and b["status"] == 2
+        assert abs(a["energy"] - b["energy"]) < 1e-8

@@ -190,3 +190,34 @@ def test_tile_shapes_reach_the_cohorts_that_start_after(lib):
     assert L.stub_shapes_set() == 2
     assert all(poll(L, h, i)[0] == 2 for i in ids)
     L.gtf_server_destroy(h)
+
+
+def test_tile_shapes_by_molecule_size(lib):
+    L = lib
+    L.stub_reset()
+    L.gtf_server_set_tile_shapes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+    L.gtf_server_set_tile_shapes_range.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.POINTER(ctypes.c_int)]
+    L.stub_shape_units.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+
+    def shapes(units):
+        return (ctypes.c_int * (21 * 6))(*([units, 64, 65536, 2048, 0, 0] * 21))
+
+    h = server(L, 1e12, 1, 1)  # one job per cohort, one cohort at a time
+    assert L.gtf_server_set_tile_shapes(h, shapes(111)) == 0            # every size
+    assert L.gtf_server_set_tile_shapes_range(h, 0, 47, shapes(222)) == 0
+    assert L.gtf_server_set_tile_shapes_range(h, 400, 100000, shapes(333)) == 0
+    assert L.gtf_server_set_tile_shapes_range(h, 50, 10, shapes(444)) == -3  # empty range
+    mols = [L.stub_molecule(n, 0.01, -1.0) for n in (30, 100, 800)]
+    for m in mols:
+        L.gtf_server_submit(h, m)
+        L.gtf_server_wait(h)
+    out = (ctypes.c_int * 8)()
+    n = L.stub_shape_units(out, 8)
+    assert list(out[:n]) == [222, 111, 333]
+    # clearing: later cohorts run with the built-in defaults
+    assert L.gtf_server_set_tile_shapes(h, None) == 0
+    L.gtf_server_submit(h, mols[2])
+    L.gtf_server_wait(h)
+    assert L.stub_shape_units(out, 8) == 3
+    L.gtf_server_destroy(h)
