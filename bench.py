@@ -409,7 +409,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--serve-rings", type=int, default=3, help="largest graphene flake (rings) in the request stream")
     ap.add_argument("--serve-budget", type=float, default=64.0, help="HBM budget of the server, GiB")
-    ap.add_argument("--serve-cohorts", type=int, default=4)
+    ap.add_argument("--serve-cohorts", type=int, default=16,
+                    help="cohorts the server runs at once (16: a request starts as soon as it arrives at this load; "
+                         "4 queued small requests behind flake cohorts, p50 9.6 s instead of 1.7-2.5 s)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--scf-compare", type=int, default=0,
                     help="SCF iterations of a per-iteration FP64 vs mixed energy comparison on the workload molecule")
