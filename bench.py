@@ -302,7 +302,7 @@ def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters, tune_nmol, tune_reps, t
     return out
 
 
-def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max_iters, ref, shapes=()):
+def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max_iters, ref, shapes=(), large=0):
     """SCF requests with seeded Poisson arrivals (rate jobs/s) submitted in
     real time to the request server (mixed precision, HBM-budget admission,
     several cohorts on the device at once).  Time to converge = done time -
@@ -325,6 +325,8 @@ def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max
                    max_iters=max_iters, ediff=1e-8)
     for lo, hi, sh in shapes:
         srv.set_tile_shapes_range(lo, hi, sh)
+    if large > 0:
+        srv.set_large_slots(large)
     t0 = time.time()
     ids = []
     for b, at in zip(bases, arrivals):
@@ -411,7 +413,9 @@ def main():
     ap.add_argument("--serve-budget", type=float, default=64.0, help="HBM budget of the server, GiB")
     ap.add_argument("--serve-cohorts", type=int, default=16,
                     help="cohorts the server runs at once (16: a request starts as soon as it arrives at this load; "
-                         "4 queued small requests behind flake cohorts, p50 9.6 s instead of 1.7-2.5 s)")
+                         "4 queued small requests behind flake cohorts, p50 9.6 s instead of 1.3-2.5 s)")
+    ap.add_argument("--serve-large", type=int, default=2,
+                    help="of those, cohorts of molecules with 384 or more basis functions")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--scf-compare", type=int, default=0,
                     help="SCF iterations of a per-iteration FP64 vs mixed energy comparison on the workload molecule")
@@ -572,7 +576,8 @@ def main():
                     ranges.append((lo, hi, sh))
                     used.append("%s for nbf %d.." % (name, lo))
         served = serve_leg(g, gbs, args.serve, args.serve_rate, 20261203, args.tau, args.serve_rings,
-                           args.serve_budget, args.serve_cohorts, 40, True, ranges)
+                           args.serve_budget, args.serve_cohorts, 40, True, ranges, args.serve_large)
+        served["large_slots"] = args.serve_large
         served["tile_shapes"] = ("tuning tables: " + ", ".join(used)) if used else "built-in defaults"
 
     batched = None

@@ -125,6 +125,7 @@ def lib():
         "gtf_batch_set_tile_shapes": ([_vp, _ip], ctypes.c_int),
         "gtf_server_set_tile_shapes": ([_vp, _ip], ctypes.c_int),
         "gtf_server_set_tile_shapes_range": ([_vp, ctypes.c_int, ctypes.c_int, _ip], ctypes.c_int),
+        "gtf_server_set_large_slots": ([_vp, ctypes.c_int], ctypes.c_int),
         "gtf_batch_autotune": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
         "gtf_batch_tile_shape": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
         "gtf_batch_set_tile_shape": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
@@ -638,6 +639,12 @@ class Server:
             st = lib().gtf_server_set_tile_shapes(self.h, sh.ctypes.data_as(_ip))
         if st != 0:
             raise GtfError("gtf_server_set_tile_shapes failed (%d)" % st)
+
+    def set_large_slots(self, n):
+        """Cohorts of molecules with 384 or more basis functions run at once."""
+        st = lib().gtf_server_set_large_slots(self.h, int(n))
+        if st != 0:
+            raise GtfError("gtf_server_set_large_slots failed (%d)" % st)
 
     def set_tile_shapes_range(self, nbf_min, nbf_max, shapes):
         """Shapes for cohorts whose largest molecule has nbf_min..nbf_max basis

@@ -273,6 +273,15 @@ int gtf_server_create(double hbm_budget_bytes, int max_cohorts, int max_jobs_per
     return 0;
 }
 
+int gtf_server_set_large_slots(gtf_server_t s, int n)
+{
+    if (!s || n < 1) return -3;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->max_large = std::min(n, s->max_cohorts);
+    s->cv.notify_all();
+    return 0;
+}
+
 int gtf_server_set_tile_shapes(gtf_server_t s, const int* shapes)
 {
     if (!s) return -3;

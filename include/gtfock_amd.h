@@ -198,6 +198,10 @@ typedef struct GtfServer *gtf_server_t;
 
 int gtf_server_create(double hbm_budget_bytes, int max_cohorts, int max_jobs_per_cohort, double tolscr, double tau,
                       int max_iters, double ediff, gtf_server_t *server);
+/* cohorts of molecules of 384 or more basis functions that may run at once
+ * (default: half of max_cohorts, at least 1; at most max_cohorts).  Returns
+ * 0 or -3. */
+int gtf_server_set_large_slots(gtf_server_t server, int n);
 /* tile shapes (21 x GTF_TILE_SHAPE_FIELDS, as gtf_batch_set_tile_shapes) for the cohorts that
  * start after the call, e.g. the autotuner's on a sample of the request
  * distribution; NULL: the built-in defaults.  Returns 0 or -3. */

@@ -221,3 +221,25 @@ def test_tile_shapes_by_molecule_size(lib):
     L.gtf_server_wait(h)
     assert L.stub_shape_units(out, 8) == 3
     L.gtf_server_destroy(h)
+
+
+def test_large_slots_setting(lib):
+    """With 8 cohort slots and one large slot, large molecules run one at a
+    time while small ones use the other slots."""
+    L = lib
+    L.stub_reset()
+    L.gtf_server_set_large_slots.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    h = server(L, 1e13, 8, 1)
+    assert L.gtf_server_set_large_slots(h, 0) == -3
+    assert L.gtf_server_set_large_slots(h, 1) == 0
+    big = [L.stub_molecule(800, 0.2, -10.0) for _ in range(3)]
+    small = [L.stub_molecule(30, 0.05, -1.0) for _ in range(4)]
+    ids_big = [L.gtf_server_submit(h, b) for b in big]
+    ids_small = [L.gtf_server_submit(h, b) for b in small]
+    L.gtf_server_wait(h)
+    spans = sorted((poll(L, h, i)[1][3], poll(L, h, i)[1][4]) for i in ids_big)
+    # one large cohort at a time: each starts after the previous one ended
+    assert spans[1][0] >= spans[0][1] - 0.01 and spans[2][0] >= spans[1][1] - 0.01
+    done_small = max(poll(L, h, i)[1][4] for i in ids_small)
+    assert done_small < spans[0][1]  # the small jobs did not wait for them
+    L.gtf_server_destroy(h)
