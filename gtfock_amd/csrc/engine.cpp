@@ -510,7 +510,7 @@ int GtfSystem::build_tiles()
 // predicates per quartet.
 int GtfSystem::build_class_tiles(int cls)
 {
-    const double tau2 = tau * tau;
+    const double tau2 = class_tau2(cls);
     {
         int A, B;
         gtf_class_types(cls, A, B);
@@ -683,7 +683,7 @@ int GtfSystem::launch_class(int cls, hipStream_t cs, const double* D, double* J,
     a.row_kstart = d_row_kstart[cls];
     if (ntiles[cls] > 0 || ntiles_lo[cls] > 0) {
         a.nbf = nbf;
-        a.tau2 = tau * tau;
+        a.tau2 = class_tau2(cls);
         const bool exch = exchanged(cls);
         const int grid = shape[cls].grid;
         const int bl = shape[cls].boys_lds ? 1 : 0;
@@ -807,6 +807,11 @@ int GtfSystem::autotune(const double* D, double* J, double* K, int reps, double*
                  return true;
              })))
             break;
+        if (tau > 0.0 && (rc = coordinate(1, [&](int) {
+                              shape[cls].fp32 ^= 1;
+                              return true;
+                          })))
+            break;
         static const int units[] = {256, 512, 1024, 2048, 4096};
         if ((rc = coordinate(5, [&](int k) {
                  if (shape[cls].units == units[k]) return false;
@@ -864,13 +869,16 @@ int GtfSystem::get_shape(int cls, int* out) const
     out[3] = shape[cls].grid;
     out[4] = exchanged(cls) ? 1 : 0;
     out[5] = shape[cls].boys_lds;
+    out[6] = shape[cls].fp32;
     return GTF_SHAPE_FIELDS;
 }
 
 static bool shape_valid(int cls, const int* in)
 {
-    if (in[0] < 1 || in[1] < 1 || in[2] < 1 || in[3] < 1 || in[4] < 0 || in[4] > 1 || in[5] < 0 || in[5] > 1)
+    if (in[0] < 1 || in[1] < 1 || in[2] < 1 || in[3] < 1)
         return false;
+    for (int f = 4; f < GTF_SHAPE_FIELDS; f++)
+        if (in[f] < 0 || in[f] > 1) return false;
     return !(in[4] == 1 && gtf_eri_class(cls)->nsplit_t == 0);
 }
 
@@ -882,6 +890,7 @@ void GtfSystem::put_shape(int cls, const int* in)
     shape[cls].grid = in[3];
     exchanged_mask = in[4] ? exchanged_mask | (1u << cls) : exchanged_mask & ~(1u << cls);
     shape[cls].boys_lds = in[5];
+    shape[cls].fp32 = in[6];
 }
 
 int GtfSystem::set_shape(int cls, const int* in)

@@ -18,7 +18,7 @@
 #include "gtf_internal.h"
 
 // ints per class in the tile-shape entry points
-#define GTF_SHAPE_FIELDS 6
+#define GTF_SHAPE_FIELDS 7
 
 struct GtfSystem {
     hipStream_t stream = nullptr;
@@ -103,8 +103,11 @@ struct GtfSystem {
     // `units` work units and about `tile_pq` primitive quartets per tile,
     // about `unit_pq` primitive quartets per unit (how finely a quartet's bra
     // primitives are chunked), `grid` persistent workgroups per launch, each
-    // taking tiles from the class counter until none are left, and
-    // `boys_lds` whether the workgroups stage the Boys slab in LDS.  The
+    // taking tiles from the class counter until none are left,
+    // `boys_lds` whether the workgroups stage the Boys slab in LDS, and
+    // `fp32` whether the class evaluates its quartets below the Schwarz
+    // bound tau in FP32 (with 0 it stays FP64 throughout, which is never
+    // less accurate; only the speed differs from class to class).  The
     // defaults are the shapes measured best on C10H22 / C54H18 by hand
     // (2048 persistent workgroups beat one workgroup per tile: C10H22 7.6 vs
     // 7.9 ms, C54H18 404 vs 433 ms).
@@ -114,6 +117,7 @@ struct GtfSystem {
         int tile_pq = GTF_TILE_PQ;
         int grid = 2048;
         int boys_lds = 0;
+        int fp32 = 1;
     };
     TileShape shape[GTF_NCLASSES];
     // classes run by the bra/ket-exchanged tile kernel (bit = class index;
@@ -125,6 +129,8 @@ struct GtfSystem {
     bool exchanged(int cls) const;
     // mixed precision: Schwarz bound threshold tau (0 = plain FP64)
     double tau = 0.0;
+    // squared threshold of a class (0 when the class stays FP64)
+    double class_tau2(int cls) const { return shape[cls].fp32 ? tau * tau : 0.0; }
     long nq_lo[GTF_NCLASSES] = {0};
     double nprimq_lo[GTF_NCLASSES] = {0};
     int* d_counters = nullptr;
@@ -175,7 +181,7 @@ struct GtfSystem {
     int autotune(const double* D, double* J, double* K, int reps, double* ms_out);
     int time_class(int cls, const double* D, double* J, double* K, int reps, float* ms);
     // a class's shape as {units, unit_pq, tile_pq, grid, exchanged,
-    // boys_lds} (GTF_SHAPE_FIELDS ints); set
+    // boys_lds, fp32} (GTF_SHAPE_FIELDS ints); set
     // rebuilds the class's tiles (exchanged = 1 only where the class has an
     // exchanged kernel); set returns -3 for an invalid shape, -1 on a HIP
     // failure

@@ -4,13 +4,13 @@ with those shapes later without tuning again.
 
 One JSON file per workload under gtfock_amd/data/tuning/:
 {"workload": name, "shapes": [21 x {units, unit_pq, tile_pq, grid,
-exchanged, boys_lds}] in quartet-class order, "tuned": {how and where it was tuned,
+exchanged, boys_lds, fp32}] in quartet-class order, "tuned": {how and where it was tuned,
 and the autotuner's own before/after build times}}.
 """
 import json
 import os
 
-SHAPE_KEYS = ("units", "unit_pq", "tile_pq", "grid", "exchanged", "boys_lds")
+SHAPE_KEYS = ("units", "unit_pq", "tile_pq", "grid", "exchanged", "boys_lds", "fp32")
 NCLASSES = 21
 TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tuning")
 

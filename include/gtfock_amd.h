@@ -75,13 +75,14 @@ double gtf_pfock_device_bytes(PFock_t pfock);
 
 /* Tile autotuner.  The tile launches of a quartet class have a shape
  * shape[GTF_TILE_SHAPE_FIELDS] = {units, unit_pq, tile_pq, grid, exchanged,
- * boys_lds}: at most `units` work units and about `tile_pq` primitive
+ * boys_lds, fp32}: at most `units` work units and about `tile_pq` primitive
  * quartets per tile, about `unit_pq` primitive quartets per work unit (how
  * finely a quartet's bra primitives are split), `grid` persistent
  * workgroups, exchanged = 1 for the bra/ket-exchanged kernel (only for
  * classes that have one), boys_lds = 1 for the kernels that stage the Boys
- * table in LDS. */
-#define GTF_TILE_SHAPE_FIELDS 6
+ * table in LDS, fp32 = 1 to evaluate the class's quartets below the mixed-
+ * precision threshold tau in FP32 (0: the class stays FP64). */
+#define GTF_TILE_SHAPE_FIELDS 7
 /* gtf_pfock_autotune times the classes, each alone, over a coordinate
  * search of these (best of reps launches per candidate) with density 0 as
  * last put, then keeps the tuned shapes only if the whole build is faster

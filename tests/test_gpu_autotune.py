@@ -63,6 +63,7 @@ def test_every_tile_shape_matches_oracle(g, gbs, butane, name):
     for cls in range(21):
         sh = dict(EXTREME_SHAPES[name])
         sh["exchanged"] = pf.tile_shape(cls)["exchanged"]
+        sh["fp32"] = cls % 2  # every other class stays FP64 in the mixed build below
         pf.set_tile_shape(cls, sh)
         assert pf.tile_shape(cls) == sh
     print("shape %s, case butane seed 31" % name, flush=True)
@@ -107,6 +108,7 @@ def test_invalid_tile_shapes_are_rejected(g, gbs, butane):
     pf = g.PFock(b, tolscr=1e-11)
     good = pf.tile_shape(0)
     for bad in [dict(good, units=0), dict(good, grid=-1), dict(good, exchanged=2), dict(good, boys_lds=2),
+                dict(good, fp32=-1),
                 dict(good, exchanged=1)]:  # class 0, (ss|ss), has no exchanged kernel
         with pytest.raises(g.GtfError):
             pf.set_tile_shape(0, bad)
@@ -181,4 +183,29 @@ def test_committed_tuning_tables_match_oracle(g, gbs, butane):
         pf.set_tile_shapes(shapes)
         assert pf.tile_shapes() == shapes
         assert np.abs(pf.fock(D) - Fr).max() < FOCK_TOL
+    pf.close()
+
+
+@gpu
+def test_fp32_off_keeps_classes_in_fp64(g, gbs, butane):
+    """With fp32 = 0 on every class a mixed-precision build evaluates no
+    quartet in FP32 and equals the FP64 build; with it on again the FP32
+    share returns."""
+    xyz, D, Fr = butane
+    b = g.BasisSet(gbs, xyz)
+    pf = g.PFock(b, tolscr=1e-11)
+    F64 = pf.fock(D)
+    shapes = pf.tile_shapes()
+    pf.set_precision(1e-3)
+    nq_lo, _ = pf.mixed_counts()
+    assert nq_lo.sum() > 0
+    pf.set_tile_shapes([dict(sh, fp32=0) for sh in shapes])
+    nq_lo, _ = pf.mixed_counts()
+    assert nq_lo.sum() == 0
+    F = pf.fock(D)
+    assert np.abs(F - F64).max() < ORDER_TOL
+    assert np.abs(F - Fr).max() < FOCK_TOL
+    pf.set_tile_shapes(shapes)
+    nq_lo, _ = pf.mixed_counts()
+    assert nq_lo.sum() > 0
     pf.close()

@@ -179,7 +179,7 @@ def test_tile_shapes_reach_the_cohorts_that_start_after(lib):
     ids = [L.gtf_server_submit(h, mols[0])]
     L.gtf_server_wait(h)
     assert L.stub_shapes_set() == 0  # built-in defaults: nothing set
-    shapes = (ctypes.c_int * (21 * 6))(*([777, 64, 65536, 2048, 0, 0] * 21))
+    shapes = (ctypes.c_int * (21 * 7))(*([777, 64, 65536, 2048, 0, 0, 1] * 21))
     assert L.gtf_server_set_tile_shapes(h, shapes) == 0
     ids += [L.gtf_server_submit(h, m) for m in mols[1:3]]
     L.gtf_server_wait(h)
@@ -201,7 +201,7 @@ def test_tile_shapes_by_molecule_size(lib):
     L.stub_shape_units.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
 
     def shapes(units):
-        return (ctypes.c_int * (21 * 6))(*([units, 64, 65536, 2048, 0, 0] * 21))
+        return (ctypes.c_int * (21 * 7))(*([units, 64, 65536, 2048, 0, 0, 1] * 21))
 
     h = server(L, 1e12, 1, 1)  # one job per cohort, one cohort at a time
     assert L.gtf_server_set_tile_shapes(h, shapes(111)) == 0            # every size
