@@ -697,11 +697,31 @@ class PurifT(ctypes.Structure):
         ("timepass", ctypes.c_double), ("dev", _vp)]
 
 
-def scf(gbs, xyz, niters, tolscr=1e-11, verbose=False, tau=0.0, purif_fp32=False):
-    """pscf/scf.c:189-474 on one GPU with a zero initial density
-    (core-Hamiltonian guess).  tau > 0 selects the mixed-precision Fock
-    build, purif_fp32 the mixed-precision purification.  Returns
-    (energies, last D, last F = H + G(D))."""
+def initial_guess(basis):
+    """The starting density of pscf/scf.c:34-78: the atomic blocks of
+    CInt_getInitialGuess (read from <Element>.dat next to the basis file;
+    zero where there is none) on the block diagonal, times R/2 with
+    R = (N_neutral - Q) / N_neutral for a molecule of charge Q."""
+    L = lib()
+    n = basis.nbf
+    D = np.zeros((n, n))
+    for a in range(basis.natoms):
+        g = ctypes.POINTER(ctypes.c_double)()
+        s, e = ctypes.c_int(), ctypes.c_int()
+        L.CInt_getInitialGuess(basis.h, a, ctypes.byref(g), ctypes.byref(s), ctypes.byref(e))
+        m = e.value - s.value + 1
+        D[s.value:e.value + 1, s.value:e.value + 1] = np.ctypeslib.as_array(g, shape=(m * m,)).reshape(m, m)
+    nn, q = L.CInt_getNneutral(basis.h), L.CInt_getTotalCharge(basis.h)
+    R = (nn - q) / float(nn) if (q != 0 and nn != 0) else 1.0
+    return D * (R / 2.0)
+
+
+def scf(gbs, xyz, niters, tolscr=1e-11, verbose=False, tau=0.0, purif_fp32=False, guess=False):
+    """pscf/scf.c:189-474 on one GPU.  The initial density is zero
+    (core-Hamiltonian guess), or with guess=True the atomic-block guess of
+    scf.c's initial_guess (zero for atoms without a .dat file).  tau > 0
+    selects the mixed-precision Fock build, purif_fp32 the mixed-precision
+    purification.  Returns (energies, last D, last F = H + G(D))."""
     basis = BasisSet(gbs, xyz)
     pf = PFock(basis, tolscr=tolscr)
     if tau > 0:
@@ -714,7 +734,7 @@ def scf(gbs, xyz, niters, tolscr=1e-11, verbose=False, tau=0.0, purif_fp32=False
     H = pf.core_hamiltonian()
     pu.set_xs(X, S)
     n = basis.nbf
-    D = np.zeros((n, n))
+    D = initial_guess(basis) if guess else np.zeros((n, n))
     enuc = basis.nuc_energy()
     e0 = -1.0
     energies = []

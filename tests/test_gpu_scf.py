@@ -3,6 +3,8 @@
 Bars: purified density within 1e-10 of the oracle's (same algorithm, FP64);
 total energies within 1e-9 Eh of the oracle for every SCF iteration.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -152,3 +154,21 @@ def test_mixed_purification_projector(g, gbs, mol_dir):
     assert np.abs(D32 - D64).max() < 1e-4
     p64.close()
     p32.close()
+
+
+@gpu
+def test_scf_from_atomic_guess_reaches_the_same_energy(gbs, mol_dir, tmp_path):
+    """scf.c's atomic-block starting density (the reference's .dat guesses)
+    converges to the RHF energy the zero-density start reaches."""
+    import shutil
+    import gtfock_amd as g
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "guess")
+    for f in ("H.dat", "C.dat", "O.dat"):
+        shutil.copy(os.path.join(golden, f), str(tmp_path / f))
+    shutil.copy(gbs, str(tmp_path / "cc-pvdz.gbs"))
+    xyz = str(mol_dir / "water.xyz")
+    e0, _, _ = g.scf(gbs, xyz, 30)
+    e1, _, _ = g.scf(str(tmp_path / "cc-pvdz.gbs"), xyz, 30, guess=True)
+    print("water: zero start %.10f (%d builds), atomic guess %.10f (%d builds)" % (e0[-1], len(e0), e1[-1], len(e1)))
+    assert abs(e1[-1] - e0[-1]) < 1e-8
+    assert abs(e1[-1] - (-76.0267987)) < 1e-6
