@@ -87,6 +87,7 @@ struct GtfBatch {
     int enqueue_fock();
     int fock();
     int scf_setup();
+    int put_initial_guess();
     int diis(int iter, const std::vector<int>& active);
     int purify(const std::vector<int>& active);
     void invalidate_graph()
@@ -171,6 +172,39 @@ int GtfBatch::fock()
     last_ms = ms;
     if (sys.collect_times()) return -1;
     nbuilds++;
+    return 0;
+}
+
+// The starting density of pscf/scf.c:34-78 for every molecule: its atomic
+// guess blocks (CInt_getInitialGuess; zero for atoms whose basis directory
+// has no <Element>.dat) on the block diagonal, times R/2 with
+// R = (N_neutral - Q) / N_neutral.  D must already be zero; nothing is
+// copied when every block is zero.
+int GtfBatch::put_initial_guess()
+{
+    std::vector<double> h;
+    for (int m = 0; m < nmol; m++) {
+        BasisSet_t bs = const_cast<BasisSet_t>(bases[m]);
+        const int n = this->n[m];
+        const int nn = CInt_getNneutral(bs), q = CInt_getTotalCharge(bs);
+        const double r2 = 0.5 * ((q != 0 && nn != 0) ? (double)(nn - q) / nn : 1.0);
+        for (int a = 0; a < CInt_getNumAtoms(bs); a++) {
+            double* g = nullptr;
+            int s0 = 0, s1 = -1;
+            CInt_getInitialGuess(bs, a, &g, &s0, &s1);
+            const int ld = s1 - s0 + 1;
+            for (int i = 0; i < ld; i++)
+                for (int j = 0; j < ld; j++) {
+                    const double v = g[i * ld + j];
+                    if (v == 0.0) continue;
+                    if (h.empty()) h.assign((size_t)total, 0.0);
+                    h[(size_t)off[m] + (size_t)(s0 + i) * n + s0 + j] = v * r2;
+                }
+        }
+    }
+    if (h.empty()) return 0;
+    B_CHECK(hipMemcpyAsync(d_D, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, stream));
+    B_CHECK(hipStreamSynchronize(stream));
     return 0;
 }
 
@@ -658,6 +692,7 @@ int gtf_batch_scf_cb(gtf_batch_t b, int max_iters, double ediff, double* energy,
             for (int j = 0; j <= B_MAX_DIIS; j++) b->b_mat[(size_t)m * LB * LB + i * LB + j] = (i != j) ? -1.0 : 0.0;
     b->time_fock = b->time_diis = b->time_purif = 0.0;
     if (gtf_fill(b->stream, b->d_D, b->total, 0.0)) return -1;
+    if (b->put_initial_guess()) return -1;
     std::vector<int> active(nmol, 1), skip(nmol, 0), nit(nmol, max_iters);
     std::vector<double> e0(nmol, -1.0), e(nmol, 0.0);
     const BatchLayout L = b->layout();

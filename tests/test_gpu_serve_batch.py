@@ -218,3 +218,30 @@ def test_server_tuned_shapes_keep_energies(g, gbs, small_batch):
             k, bases[k].nbf, a["energy"], a["iters"], b["energy"], b["iters"]))
         assert a["status"] == 2 and b["status"] == 2
         assert abs(a["energy"] - b["energy"]) < 1e-8
+
+
+@gpu
+def test_batched_scf_starts_from_the_atomic_guess(g, gbs, mol_dir, tmp_path):
+    """With <Element>.dat guesses beside the basis file the batched SCF
+    starts, as scf.c does, from the atomic blocks (R/2 scaled) and matches
+    the single-molecule SCF from the same start, molecule by molecule."""
+    import os
+    import shutil
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "guess")
+    for f in ("H.dat", "C.dat", "O.dat"):
+        shutil.copy(os.path.join(golden, f), str(tmp_path / f))
+    basis_file = str(tmp_path / "cc-pvdz.gbs")
+    shutil.copy(gbs, basis_file)
+    xyzs = [str(mol_dir / (n + ".xyz")) for n in ("water", "methane", "ethane")]
+    bt = g.Batch([g.BasisSet(basis_file, x) for x in xyzs])
+    e, it = bt.scf(max_iters=30, ediff=1e-11)
+    bt.close()
+    bz = g.Batch([g.BasisSet(gbs, x) for x in xyzs])
+    ez, itz = bz.scf(max_iters=30, ediff=1e-11)
+    bz.close()
+    for k, x in enumerate(xyzs):
+        es, _, _ = g.scf(basis_file, x, 30, guess=True)
+        print("%s: batch from guess %.10f (%d builds), single from guess %.10f (%d), batch from zero %.10f (%d)" % (
+            os.path.basename(x), e[k], it[k], es[-1], len(es), ez[k], itz[k]))
+        assert abs(e[k] - es[-1]) < 1e-9
+        assert abs(e[k] - ez[k]) < 1e-8
