@@ -417,6 +417,9 @@ def main():
     ap.add_argument("--serve-large", type=int, default=2,
                     help="of those, cohorts of molecules with 384 or more basis functions")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--guess", choices=["atomic", "zero"], default="atomic",
+                    help="starting density of the batched and served SCFs: scf.c's atomic blocks, generated by "
+                         "gtfock_amd.guess, or zero")
     ap.add_argument("--scf-compare", type=int, default=0,
                     help="SCF iterations of a per-iteration FP64 vs mixed energy comparison on the workload molecule")
     args = ap.parse_args()
@@ -561,6 +564,18 @@ def main():
                    "dE_last_iterations_max_Eh": float(d[len(d) // 2:].max()),
                    "E_change_last_iteration_fp64_Eh": float(abs(h64[0][-1] - h64[0][-2]))}
 
+    # basis directory of the SCF legs: the basis file with the library's
+    # atomic guesses beside it (CInt_getInitialGuess reads them from there)
+    scf_gbs = gbs
+    if args.guess == "atomic" and (args.serve > 0 or args.batch > 0):
+        import shutil
+        from gtfock_amd import guess
+        log("atomic guesses for H, C, O")
+        gdir = tempfile.mkdtemp()
+        scf_gbs = os.path.join(gdir, os.path.basename(gbs))
+        shutil.copy(gbs, scf_gbs)
+        guess.write_dat(gbs, gdir, ("H", "C", "O"))
+
     served = None
     if args.serve > 0:
         log("serve leg: %d jobs at %.0f jobs/s" % (args.serve, args.serve_rate))
@@ -575,16 +590,18 @@ def main():
                 if sh is not None:
                     ranges.append((lo, hi, sh))
                     used.append("%s for nbf %d.." % (name, lo))
-        served = serve_leg(g, gbs, args.serve, args.serve_rate, 20261203, args.tau, args.serve_rings,
+        served = serve_leg(g, scf_gbs, args.serve, args.serve_rate, 20261203, args.tau, args.serve_rings,
                            args.serve_budget, args.serve_cohorts, 40, True, ranges, args.serve_large)
         served["large_slots"] = args.serve_large
+        served["initial_guess"] = args.guess
         served["tile_shapes"] = ("tuning tables: " + ", ".join(used)) if used else "built-in defaults"
 
     batched = None
     if args.batch > 0:
         log("batched leg: %d molecules" % args.batch)
-        batched = batch_leg(g, gbs, args.batch, args.batch_seed, args.tau, max(2, args.steps // 4), args.batch_scf,
+        batched = batch_leg(g, scf_gbs, args.batch, args.batch_seed, args.tau, max(2, args.steps // 4), args.batch_scf,
                             args.batch_tune, max(1, args.autotune_reps - 1), tiles)
+        batched["initial_guess"] = args.guess
 
     cpu = None
     if not args.no_cpu:

@@ -172,3 +172,39 @@ def test_scf_from_atomic_guess_reaches_the_same_energy(gbs, mol_dir, tmp_path):
     print("water: zero start %.10f (%d builds), atomic guess %.10f (%d builds)" % (e0[-1], len(e0), e1[-1], len(e1)))
     assert abs(e1[-1] - e0[-1]) < 1e-8
     assert abs(e1[-1] - (-76.0267987)) < 1e-6
+
+
+@gpu
+def test_generated_atomic_guesses(gbs, mol_dir, tmp_path):
+    """The library's own atomic guesses (gtfock_amd.guess): a fractional,
+    spherically averaged SCF of each free atom.  Each block holds Z
+    electrons (tr(D S) = Z) and an SCF started from them reaches the
+    zero-start energy in fewer builds.  The distance to the reference's
+    .dat file for the element is printed, not asserted: those come from
+    another atomic method (up to 0.3 apart element-wise)."""
+    import shutil
+    import gtfock_amd as g
+    from gtfock_amd import guess
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "guess")
+    basis_file = str(tmp_path / "cc-pvdz.gbs")
+    shutil.copy(gbs, basis_file)
+    guess.write_dat(gbs, str(tmp_path), elements=("H", "C", "O"))
+    for el, z in (("H", 1), ("C", 6), ("O", 8)):
+        ours = np.loadtxt(str(tmp_path / (el + ".dat")))
+        ref = np.loadtxt(os.path.join(golden, el + ".dat"))
+        n = int(round(len(ours) ** 0.5))
+        d = tmp_path / ("atom_" + el)
+        d.mkdir()
+        (d / "a.xyz").write_text("1\n\n%s 0 0 0\n" % el)
+        S = oracle.Basis(gbs, str(d / "a.xyz")).overlap()
+        tr = float(np.trace(ours.reshape(n, n) @ S))
+        print("%s: tr(DS) %.8f, max |ours - reference .dat| %.3e" % (el, tr, np.abs(ours - ref).max()))
+        assert len(ours) == len(ref)
+        assert abs(tr - z) < 1e-8
+    for name in ("water", "ethane"):
+        xyz = str(mol_dir / (name + ".xyz"))
+        e0, _, _ = g.scf(gbs, xyz, 30)
+        e1, _, _ = g.scf(basis_file, xyz, 30, guess=True)
+        print("%s: zero start %d builds, generated guess %d builds" % (name, len(e0), len(e1)))
+        assert abs(e1[-1] - e0[-1]) < 1e-8
+        assert len(e1) < len(e0)
