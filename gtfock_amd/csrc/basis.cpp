@@ -246,6 +246,10 @@ CIntStatus_t CInt_loadBasisSet(BasisSet_t b, char* bsfile, char* xyzfile)
         int nfa = 0;
         for (int s = b->s_startind[a]; s < b->s_startind[a + 1]; s++) nfa += 2 * b->L[s] + 1;
         read_guess(dir, z, b->guess[z], nfa);
+        // no (usable) file: a zero block, sized here so that
+        // CInt_getInitialGuess only reads (batches on several threads may
+        // share a basis)
+        if (b->guess[z].empty()) b->guess[z].assign((size_t)nfa * nfa, 0.0);
     }
     return CINT_STATUS_SUCCESS;
 }

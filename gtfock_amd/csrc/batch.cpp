@@ -184,15 +184,20 @@ int GtfBatch::put_initial_guess()
 {
     std::vector<double> h;
     for (int m = 0; m < nmol; m++) {
-        BasisSet_t bs = const_cast<BasisSet_t>(bases[m]);
+        const BasisSet* bs = bases[m];
         const int n = this->n[m];
-        const int nn = CInt_getNneutral(bs), q = CInt_getTotalCharge(bs);
+        const int nn = bs->nelec_neutral, q = bs->charge;  // CInt_getNneutral, CInt_getTotalCharge
         const double r2 = 0.5 * ((q != 0 && nn != 0) ? (double)(nn - q) / nn : 1.0);
-        for (int a = 0; a < CInt_getNumAtoms(bs); a++) {
-            double* g = nullptr;
-            int s0 = 0, s1 = -1;
-            CInt_getInitialGuess(bs, a, &g, &s0, &s1);
+        for (int a = 0; a < bs->natoms; a++) {
+            // the block CInt_getInitialGuess returns, read in place (that
+            // call may size a missing block, and several cohort threads can
+            // hold one basis)
+            const int s0 = bs->fstart[bs->s_startind[a]], s1 = bs->fstart[bs->s_startind[a + 1]] - 1;
             const int ld = s1 - s0 + 1;
+            if (bs->Z[a] >= (int)bs->guess.size()) continue;
+            const std::vector<double>& gv = bs->guess[bs->Z[a]];
+            if ((int)gv.size() != ld * ld) continue;  // no guess: zero block
+            const double* g = gv.data();
             for (int i = 0; i < ld; i++)
                 for (int j = 0; j < ld; j++) {
                     const double v = g[i * ld + j];
