@@ -191,3 +191,19 @@ def test_atomic_guess_blocks_from_dat_files(g, gbs, tmp_path):
     assert (g.lib().CInt_getNneutral(b.h), g.lib().CInt_getTotalCharge(b.h)) == (9, -1)
     S = oracle.Basis(basis_file, str(xyz)).overlap()
     assert abs(np.trace(g.initial_guess(b) @ S) - 5.0) < 1e-6
+
+
+def test_bench_contract_on_the_host():
+    """bench.py refuses more than one GPU with a message (the build runs on
+    one GPU) and lists the options the benchmark contract relies on."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+    h = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--help"], capture_output=True, text=True,
+                       timeout=120)
+    assert h.returncode == 0
+    for opt in ("--gpus", "--steps", "--warmup", "--tune", "--guess", "--batch", "--serve"):
+        assert opt in h.stdout
