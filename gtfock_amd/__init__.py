@@ -126,6 +126,13 @@ def lib():
         "gtf_server_set_tile_shapes": ([_vp, _ip], ctypes.c_int),
         "gtf_server_set_tile_shapes_range": ([_vp, ctypes.c_int, ctypes.c_int, _ip], ctypes.c_int),
         "gtf_server_set_large_slots": ([_vp, ctypes.c_int], ctypes.c_int),
+        "gtf_server_set_purif_precision": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+        "gtf_server_set_join_gap": ([_vp, ctypes.c_int], ctypes.c_int),
+        "gtf_server_join_stats": ([_vp, _lp, _lp], ctypes.c_int),
+        "gtf_batch_scf_begin": ([_vp, ctypes.c_int, ctypes.c_double, _vp, _vp], ctypes.c_int),
+        "gtf_batch_scf_step": ([_vp], ctypes.c_int),
+        "gtf_batch_scf_results": ([_vp, _dp, _ip, _dp], ctypes.c_int),
+        "gtf_batch_regroup": ([_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp), _ip], ctypes.c_int),
         "gtf_batch_autotune": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
         "gtf_batch_tile_shape": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
         "gtf_batch_set_tile_shape": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
@@ -134,7 +141,10 @@ def lib():
         "gtf_gemm_host": ([ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "gtf_purif_set_precision": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "gtf_purif_fp32_iters": ([_vp], ctypes.c_int),
+        "gtf_purif_set_slices": ([_vp, ctypes.c_int], ctypes.c_int),
+        "gtf_gemm_split_host": ([ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "gtf_batch_set_purif_precision": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+        "gtf_batch_set_purif_slices": ([_vp, ctypes.c_int], ctypes.c_int),
         "create_purif": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], _vp),
         "destroy_purif": ([_vp], None),
         "compute_purification": ([_vp, _dp, _dp], ctypes.c_int),
@@ -423,15 +433,34 @@ class ERD:
         return [out[offs[i]:offs[i + 1]] for i in range(len(q))]
 
 
-def dgemm(A, B, transB=False, fp32=False):
-    """MFMA GEMM on the GPU: A @ B, or A @ B.T when transB; FP64, or FP32
-    on rounded operands when fp32."""
+def dgemm(A, B, transB=False, fp32=False, split=0):
+    """MFMA GEMM on the GPU: A @ B, or A @ B.T when transB; FP64, FP32 on
+    rounded operands when fp32, or split BF16 with `split` exact slices
+    (3..6) of every operand row when split > 0."""
     A = np.ascontiguousarray(A, dtype=np.float64)
     B = np.ascontiguousarray(B, dtype=np.float64)
     n = A.shape[0]
     C = np.zeros((n, n))
+    if split:
+        _check(lib().gtf_gemm_split_host(n, _p(A), _p(B), _p(C), 1 if transB else 0, int(split)),
+               "gtf_gemm_split_host")
+        return C
     _check(lib().gtf_gemm_host(n, _p(A), _p(B), _p(C), 1 if transB else 0, 1 if fp32 else 0), "gtf_gemm_host")
     return C
+
+
+# purification product modes of gtf_purif_set_precision / gtf_batch_set_purif_precision
+PURIF_MODES = {"fp64": 0, "fp32": 1, "split": 2}
+# ||D - D^2||_F below which the split-BF16 purification finishes in FP64
+PURIF_SPLIT_UNTIL = 1e-6
+
+
+def _purif_mode(mode):
+    if isinstance(mode, str):
+        return PURIF_MODES[mode]
+    if isinstance(mode, bool):
+        return 1 if mode else 0
+    return int(mode)
 
 
 class Purif:
@@ -465,11 +494,21 @@ class Purif:
         lib().compute_diis(pfock.h if pfock is not None else None, self.h, _p(D), _p(F), it)
         return F
 
-    def set_precision(self, fp32, fp32_until=1e-3):
-        """Mixed-precision purification products (gtf_purif_set_precision)."""
-        _check(lib().gtf_purif_set_precision(self.h, 1 if fp32 else 0, fp32_until), "gtf_purif_set_precision")
+    def set_precision(self, mode, until=None):
+        """Reduced-precision purification products (gtf_purif_set_precision):
+        mode "fp64" / "fp32" (True) / "split", until the error drops below
+        `until` (default 1e-3 for FP32, PURIF_SPLIT_UNTIL for split)."""
+        m = _purif_mode(mode)
+        if until is None:
+            until = PURIF_SPLIT_UNTIL if m == 2 else 1e-3
+        _check(lib().gtf_purif_set_precision(self.h, m, until), "gtf_purif_set_precision")
+
+    def set_slices(self, ns):
+        """Slices per operand of the split-BF16 products (3..6, default 5)."""
+        _check(lib().gtf_purif_set_slices(self.h, int(ns)), "gtf_purif_set_slices")
 
     def fp32_iters(self):
+        """iterations of the last purification run in the reduced precision"""
         return lib().gtf_purif_fp32_iters(self.h)
 
     def purify(self, F):
@@ -524,9 +563,15 @@ class Batch:
     def use_graph(self, on):
         self._st(lib().gtf_batch_use_graph(self.h, 1 if on else 0), "gtf_batch_use_graph")
 
-    def set_purif_precision(self, fp32, fp32_until=1e-3):
-        self._st(lib().gtf_batch_set_purif_precision(self.h, 1 if fp32 else 0, fp32_until),
-                 "gtf_batch_set_purif_precision")
+    def set_purif_precision(self, mode, until=None):
+        """as Purif.set_precision, per molecule of the batch"""
+        m = _purif_mode(mode)
+        if until is None:
+            until = PURIF_SPLIT_UNTIL if m == 2 else 1e-3
+        self._st(lib().gtf_batch_set_purif_precision(self.h, m, until), "gtf_batch_set_purif_precision")
+
+    def set_purif_slices(self, ns):
+        self._st(lib().gtf_batch_set_purif_slices(self.h, int(ns)), "gtf_batch_set_purif_slices")
 
     def autotune(self, reps=3):
         """Tile autotuner on the batch's density (gtf_batch_autotune)."""
@@ -600,6 +645,51 @@ class Batch:
         lib().gtf_batch_class_work(self.h, nq.ctypes.data_as(_lp), _p(npq), nq32.ctypes.data_as(_lp), _p(npq32))
         return nq, npq, nq32, npq32
 
+    def scf_begin(self, max_iters=30, ediff=1e-11):
+        """Step-wise SCF (gtf_batch_scf_begin); then scf_step() per iteration."""
+        self._max_iters = int(max_iters)
+        self._st(lib().gtf_batch_scf_begin(self.h, self._max_iters, ediff, None, None), "gtf_batch_scf_begin")
+
+    def scf_step(self):
+        """One SCF iteration of the running molecules; returns how many still run."""
+        r = lib().gtf_batch_scf_step(self.h)
+        if r < 0:
+            raise GtfError("gtf_batch_scf_step failed (%d)" % r)
+        return r
+
+    def scf_results(self):
+        """(energies, builds (0 while running), history[nmol, max_iters])"""
+        mi = self._max_iters
+        e = np.zeros(self.nmol)
+        it = np.zeros(self.nmol, dtype=np.int32)
+        hist = np.zeros(self.nmol * mi)
+        lib().gtf_batch_scf_results(self.h, _p(e), it.ctypes.data_as(_ip), _p(hist))
+        return e, it, hist.reshape(self.nmol, mi)
+
+    def regroup(self, add=()):
+        """Continuous batching (gtf_batch_regroup): a new Batch of this one's
+        running molecules, SCF state carried over, plus the BasisSets `add`
+        starting their SCF.  Returns (batch, origin); origin[k] is the index
+        in this batch, or -(j + 1) for add[j].  This batch stays valid."""
+        add = list(add)
+        arr = (_vp * max(1, len(add)))(*[b.h for b in add])
+        h = _vp()
+        origin = np.zeros(self.nmol + len(add), dtype=np.int32)
+        self._st(lib().gtf_batch_regroup(self.h, arr, len(add), ctypes.byref(h), origin.ctypes.data_as(_ip)),
+                 "gtf_batch_regroup")
+        nb = Batch.__new__(Batch)
+        nb.h = h
+        n = int(lib().gtf_batch_layout(h, None, None))
+        origin = origin[:n]
+        nb.bases = [self.bases[o] if o >= 0 else add[-o - 1] for o in origin]
+        nb.nmol = n
+        nb.nbf = np.zeros(n, dtype=np.int32)
+        nb.offset = np.zeros(n, dtype=np.int64)
+        lib().gtf_batch_layout(h, nb.nbf.ctypes.data_as(_ip), nb.offset.ctypes.data_as(_lp))
+        nb.total = int(lib().gtf_batch_total(h))
+        nb._max_iters = self._max_iters
+        return nb, origin
+
     def scf(self, max_iters=30, ediff=1e-11, history=False):
         """Batched RHF SCF; returns (energies, builds used[, history])."""
         e = np.zeros(self.nmol)
@@ -619,7 +709,7 @@ class Server:
     JOB_KEYS = ["energy", "iters", "t_submit", "t_start", "t_done", "cohort", "mem_est"]
 
     def __init__(self, hbm_budget_gb=64.0, max_cohorts=4, max_jobs_per_cohort=64, tolscr=1e-11, tau=0.0,
-                 max_iters=30, ediff=1e-8, shapes=None):
+                 max_iters=30, ediff=1e-8, shapes=None, purif="fp64", join_gap=1):
         self.h = _vp()
         st = lib().gtf_server_create(hbm_budget_gb * 2.0 ** 30, max_cohorts, max_jobs_per_cohort, tolscr, tau,
                                      max_iters, ediff, ctypes.byref(self.h))
@@ -629,6 +719,32 @@ class Server:
         self._keep = {}
         if shapes is not None:
             self.set_tile_shapes(shapes)
+        if purif != "fp64":
+            self.set_purif_precision(purif)
+        if join_gap != 1:
+            self.set_join_gap(join_gap)
+
+    def set_purif_precision(self, mode, until=None):
+        """Purification products of the cohorts started from now on, as
+        Purif.set_precision ("fp64", "fp32", "split")."""
+        m = _purif_mode(mode)
+        if until is None:
+            until = PURIF_SPLIT_UNTIL if m == 2 else 1e-3
+        st = lib().gtf_server_set_purif_precision(self.h, m, until)
+        if st != 0:
+            raise GtfError("gtf_server_set_purif_precision failed (%d)" % st)
+
+    def set_join_gap(self, iters):
+        """SCF iterations between admissions into a running cohort (< 0: off)."""
+        st = lib().gtf_server_set_join_gap(self.h, int(iters))
+        if st != 0:
+            raise GtfError("gtf_server_set_join_gap failed (%d)" % st)
+
+    def join_stats(self):
+        """(jobs admitted into running cohorts, regroups)"""
+        j, r = ctypes.c_long(), ctypes.c_long()
+        lib().gtf_server_join_stats(self.h, ctypes.byref(j), ctypes.byref(r))
+        return j.value, r.value
 
     def set_tile_shapes(self, shapes):
         """Tile shapes of the cohorts started from now on (None: defaults)."""
@@ -666,6 +782,8 @@ class Server:
         st = lib().gtf_server_poll(self.h, jid, _p(out))
         d = dict(zip(self.JOB_KEYS, out.tolist()))
         d["status"] = st
+        if st == 2 or st < 0:
+            self._keep.pop(jid, None)  # the server no longer reads the basis
         return d
 
     def wait(self):
@@ -716,19 +834,26 @@ def initial_guess(basis):
     return D * (R / 2.0)
 
 
-def scf(gbs, xyz, niters, tolscr=1e-11, verbose=False, tau=0.0, purif_fp32=False, guess=False):
+def scf(gbs, xyz, niters, tolscr=1e-11, verbose=False, tau=0.0, purif_fp32=False, guess=False, purif=None,
+        ediff=1e-11, stats=None):
     """pscf/scf.c:189-474 on one GPU.  The initial density is zero
     (core-Hamiltonian guess), or with guess=True the atomic-block guess of
     scf.c's initial_guess (zero for atoms without a .dat file).  tau > 0
-    selects the mixed-precision Fock build, purif_fp32 the mixed-precision
-    purification.  Returns (energies, last D, last F = H + G(D))."""
+    selects the mixed-precision Fock build; purif ("fp64", "fp32", "split";
+    purif_fp32=True is "fp32") the purification products.  The SCF stops
+    after the build whose energy differs from the previous one by less than
+    ediff (scf.c: 1e-11).  stats (a dict, may be None) receives the
+    purification iteration counts.  Returns (energies, last D, last F =
+    H + G(D))."""
     basis = BasisSet(gbs, xyz)
     pf = PFock(basis, tolscr=tolscr)
     if tau > 0:
         pf.set_precision(tau)
     pu = Purif(basis)
-    if purif_fp32:
-        pu.set_precision(True)
+    if purif is None:
+        purif = "fp32" if purif_fp32 else "fp64"
+    if purif != "fp64":
+        pu.set_precision(purif)
     S = pf.overlap()
     X = pf.x_matrix()
     H = pf.core_hamiltonian()
@@ -748,11 +873,14 @@ def scf(gbs, xyz, niters, tolscr=1e-11, verbose=False, tau=0.0, purif_fp32=False
         Dlast = D
         if verbose:
             print("  iter %d energy %.10f" % (it, e + enuc), flush=True)
-        if it > 0 and abs(e - e0) < 1e-11:
+        if it > 0 and abs(e - e0) < ediff:
             break
         e0 = e
         Fo = pu.diis(pf, D, F, it)
-        D, _ = pu.purify(Fo)
+        D, pit = pu.purify(Fo)
+        if stats is not None:
+            stats.setdefault("purif_iters", []).append(pit)
+            stats.setdefault("purif_low_iters", []).append(pu.fp32_iters())
     pu.close()
     pf.close()
     return np.array(energies), Dlast, F

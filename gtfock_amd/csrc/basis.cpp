@@ -263,6 +263,7 @@ CIntStatus_t CInt_importBasisSet(BasisSet_t b, int natoms, int* Zs, double* X, d
     b->pure = 1;
     b->natoms = natoms;
     for (int a = 0; a < natoms; a++) {
+        if (Zs[a] <= 0 || Zs[a] >= kNElems) return CINT_STATUS_INVALID_VALUE;
         b->Z.push_back(Zs[a]);
         b->xyz.push_back(X[a]);
         b->xyz.push_back(Y[a]);
@@ -324,6 +325,9 @@ CIntStatus_t CInt_unpackBasisSet(BasisSet_t b, void* buf)
     getvd(b->alpha); getvd(b->coef);
     b->guess.resize(geti());
     for (auto& g : b->guess) getvd(g);
+    if ((int)b->guess.size() < kNElems) b->guess.resize(kNElems);
+    for (int z : b->Z)
+        if (z <= 0 || z >= kNElems) return CINT_STATUS_INVALID_VALUE;
     finalize(b);
     return CINT_STATUS_SUCCESS;
 }
@@ -360,15 +364,22 @@ double CInt_getNucEnergy(BasisSet_t b)
     return e;
 }
 
+// all-zero block handed out for atoms without a guess (read only)
+static const int kMaxGuessDim = 256;
+static const double kZeroGuess[kMaxGuessDim * kMaxGuessDim] = {0.0};
+
 void CInt_getInitialGuess(BasisSet_t b, int atomid, double** guess, int* spos, int* epos)
 {
     int s0 = b->s_startind[atomid], s1 = b->s_startind[atomid + 1];
     *spos = b->fstart[s0];
     *epos = b->fstart[s1] - 1;
-    int nfa = *epos - *spos + 1;
-    std::vector<double>& g = b->guess[b->Z[atomid]];
-    if ((int)g.size() != nfa * nfa) g.assign((size_t)nfa * nfa, 0.0);
-    *guess = g.data();
+    const int nfa = *epos - *spos + 1;
+    const int z = b->Z[atomid];
+    // read only: several threads may hold one basis
+    if (z > 0 && z < (int)b->guess.size() && (int)b->guess[z].size() == nfa * nfa)
+        *guess = b->guess[z].data();
+    else
+        *guess = nfa <= kMaxGuessDim ? const_cast<double*>(kZeroGuess) : nullptr;
 }
 
 }  // extern "C"
@@ -378,7 +389,7 @@ void CInt_getInitialGuess(BasisSet_t b, int atomid, double** guess, int* spos, i
 // for the top order, then downward recursion, in long double.
 // the compact Taylor tables of the ERI kernels (boys_eval8): for order M and
 // grid point i, coefficient k is F_{M+k}(i * BOYS8_DELTA) / k!
-void gtf_boys_taylor_tables(double* t8, float* t8f)
+void gtf_boys_taylor_tables(double* t8)
 {
     const int ncol = BOYS8_NM + BOYS8_NTERM - 1;
     std::vector<double> F((size_t)BOYS8_NGRID * ncol);
@@ -391,7 +402,6 @@ void gtf_boys_taylor_tables(double* t8, float* t8f)
                 const double c = F[(size_t)i * ncol + M + k] / fact;
                 const size_t at = (size_t)M * BOYS8_SLAB + (size_t)i * BOYS8_NTERM + k;
                 t8[at] = c;
-                t8f[at] = (float)c;
             }
         }
 }

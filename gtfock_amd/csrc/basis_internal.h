@@ -19,4 +19,4 @@ struct BasisSet {
 
 void gtf_boys_table(double* tab, int ngrid, int ncol, double delta);
 // BOYS8_NM slabs of BOYS8_NGRID x BOYS8_NTERM Taylor coefficients (gtf_internal.h)
-void gtf_boys_taylor_tables(double* t8, float* t8f);
+void gtf_boys_taylor_tables(double* t8);

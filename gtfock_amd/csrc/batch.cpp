@@ -20,6 +20,7 @@
 #include "batch_linalg.h"
 #include "engine.h"
 #include "oneel.h"
+#include "ozaki_launch.h"
 #include "screen.h"
 
 #include <rocblas/rocblas.h>
@@ -70,25 +71,46 @@ struct GtfBatch {
     double *p_D2 = nullptr, *p_D3 = nullptr, *p_W = nullptr, *p_W2 = nullptr;  // padded work
     double *evecs = nullptr, *Fvecs = nullptr;                                  // B_MAX_DIIS padded sets
     double *d_stats = nullptr, *d_dots = nullptr, *d_coef = nullptr, *d_energy = nullptr;
-    int *d_iters = nullptr, *d_skip = nullptr, *d_nvec = nullptr, *d_slot = nullptr;
+    int *d_iters = nullptr, *d_skip = nullptr, *d_nvec = nullptr, *d_slot = nullptr, *d_skip2 = nullptr;
+    // host sources of asynchronous copies (alive until the stream syncs)
+    std::vector<int> h_skip, h_skip2, h_slot, h_nvec;
+    std::vector<double> h_coef;
     // per-molecule DIIS bookkeeping (pscf/purif.h b_mat, len_diis, bmax, bmax_id)
     std::vector<int> len_diis, bmax_id;
     std::vector<double> bmax, b_mat;
     std::vector<int> purif_iters;
+    std::vector<int> h_active;      // source of the last copy into sys.d_mol_active
     std::vector<double> done_time;  // wall clock at which each molecule's SCF ended
     double time_fock = 0, time_diis = 0, time_purif = 0;
 
     double *d_part = nullptr, *d_pcoef = nullptr;
-    // FP32 MFMA purification products while ||D - D^2||_F > purif_fp32_until
-    bool purif_fp32 = false;
-    double purif_fp32_until = 1e-3;
-    int* d_fp32 = nullptr;
+    // reduced-precision purification products while ||D - D^2||_F >
+    // purif_until, per molecule: mode 1 FP32 MFMA, mode 2 split BF16 with
+    // purif_ns slices (ozaki.h); d_low[m] holds the mode a molecule is in
+    int purif_mode = 0;
+    double purif_until = 1e-3;
+    int purif_ns = 5;
+    int* d_low = nullptr;
+    void* oz_mem = nullptr;
+    OzScratch oz;
     BatchLayout layout() const { return BatchLayout{nmol, maxn, maxpld, d_off, d_n, d_poff, d_pld, d_part, d_pcoef}; }
     int enqueue_fock();
     int fock();
     int scf_setup();
     int put_initial_guess();
-    int diis(int iter, const std::vector<int>& active);
+    int diis(const std::vector<int>& iter, const std::vector<int>& active);
+    // step-wise SCF (gtf_batch_scf_begin / _step) and its per-molecule state
+    int scf_begin(int max_iters, double ediff, gtf_scf_done_fn done, void* ctx);
+    int scf_step();
+    int scf_finish();
+    void scf_end_molecule(int m, int nit);
+    double tolscr = 1e-11;
+    int run_max_iters = 0;
+    double run_ediff = 0.0;
+    gtf_scf_done_fn run_done = nullptr;
+    void* run_ctx = nullptr;
+    std::vector<int> run_active, run_nit, run_next;  // run_next: index of the molecule's next build
+    std::vector<double> run_e0, run_energy, run_hist;
     int purify(const std::vector<int>& active);
     void invalidate_graph()
     {
@@ -232,14 +254,15 @@ int GtfBatch::scf_setup()
     d_energy = s.dalloc<double>(nmol);
     d_iters = s.dalloc<int>(nmol);
     d_skip = s.dalloc<int>(nmol);
+    d_skip2 = s.dalloc<int>(nmol);
     d_nvec = s.dalloc<int>(nmol);
     d_slot = s.dalloc<int>(nmol);
     d_part = s.dalloc<double>((size_t)nmol * BPART * BMAX_DOTS);
     d_pcoef = s.dalloc<double>(2 * (size_t)nmol);
-    d_fp32 = s.dalloc<int>(nmol);
-    if (!d_part || !d_pcoef || !d_fp32) return -2;
+    d_low = s.dalloc<int>(nmol);
+    if (!d_part || !d_pcoef || !d_low) return -2;
     if (!d_H || !d_S || !d_F || !T || !V || !evecs || !Fvecs || !d_stats || !d_dots || !d_coef || !d_energy ||
-        !d_iters || !d_skip || !d_nvec || !d_slot)
+        !d_iters || !d_skip || !d_skip2 || !d_nvec || !d_slot)
         return -2;
     for (double** p : pads)
         if (!*p) return -2;
@@ -339,18 +362,127 @@ static void binverse(int m, const double* A, int lda, double* inv)
     }
 }
 
+// RHF SCF of every molecule (pscf/scf.c:189-474), one iteration per step:
+// scf_begin resets the DIIS state and puts scf.c's initial density; each
+// scf_step builds F = H + G(D) for the molecules still running (the others'
+// quartets are skipped), takes their energies, ends those that converged
+// (|E_k - E_(k-1)| < ediff), diverged or ran out of iterations -- reporting
+// each through the callback at once -- and runs DIIS and purification for
+// the rest.  Every quantity is per molecule, so molecules may also join a
+// running SCF (gtf_batch_regroup) with their own iteration counts.
+int GtfBatch::scf_begin(int max_iters, double ediff, gtf_scf_done_fn done, void* ctx)
+{
+    if (scf_setup()) return -1;
+    const int LB = B_MAX_DIIS + 1;
+    len_diis.assign(nmol, 0);
+    bmax_id.assign(nmol, -1);
+    bmax.assign(nmol, DBL_MIN);
+    b_mat.assign((size_t)nmol * LB * LB, 0.0);
+    for (int m = 0; m < nmol; m++)
+        for (int i = 0; i <= B_MAX_DIIS; i++)
+            for (int j = 0; j <= B_MAX_DIIS; j++) b_mat[(size_t)m * LB * LB + i * LB + j] = (i != j) ? -1.0 : 0.0;
+    time_fock = time_diis = time_purif = 0.0;
+    if (gtf_fill(stream, d_D, total, 0.0)) return -1;
+    if (put_initial_guess()) return -1;
+    run_max_iters = max_iters;
+    run_ediff = ediff;
+    run_done = done;
+    run_ctx = ctx;
+    run_active.assign(nmol, 1);
+    run_nit.assign(nmol, max_iters);
+    run_next.assign(nmol, 0);
+    run_e0.assign(nmol, -1.0);
+    run_energy.assign(nmol, 0.0);
+    run_hist.assign((size_t)nmol * max_iters, NAN);
+    B_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+void GtfBatch::scf_end_molecule(int m, int nit)
+{
+    run_active[m] = 0;
+    run_nit[m] = nit;
+    done_time[m] = bwtime();
+    if (run_done) run_done(run_ctx, m, run_energy[m], nit);
+}
+
+int GtfBatch::scf_step()
+{
+    int nactive = 0;
+    for (int m = 0; m < nmol; m++) nactive += run_active[m];
+    if (nactive == 0) return 0;
+    const BatchLayout L = layout();
+    double t0 = bwtime();
+    // molecules whose SCF has ended drop out of the build (their tiles exit
+    // at once; the captured graph stays valid)
+    h_active = run_active;
+    B_CHECK(hipMemcpyAsync(sys.d_mol_active, h_active.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, stream));
+    if (fock()) return -1;
+    if (gtf_bfock_energy(stream, L, d_H, d_G, d_D, d_F, d_energy)) return -1;
+    std::vector<double> e(nmol);
+    B_CHECK(hipMemcpyAsync(e.data(), d_energy, sizeof(double) * nmol, hipMemcpyDeviceToHost, stream));
+    B_CHECK(hipStreamSynchronize(stream));
+    time_fock += bwtime() - t0;
+    std::vector<int> iter(nmol, 0);
+    nactive = 0;
+    for (int m = 0; m < nmol; m++) {
+        if (!run_active[m]) continue;
+        const int it = run_next[m]++;
+        iter[m] = it;
+        run_energy[m] = e[m] + enuc[m];
+        run_hist[(size_t)m * run_max_iters + it] = run_energy[m];
+        if (!std::isfinite(e[m])) {
+            scf_end_molecule(m, -(it + 1));  // diverged: builds < 0
+        } else if (it > 0 && fabs(e[m] - run_e0[m]) < run_ediff) {
+            scf_end_molecule(m, it + 1);
+        } else if (it == run_max_iters - 1) {
+            scf_end_molecule(m, run_max_iters);  // out of iterations
+        } else {
+            run_e0[m] = e[m];
+            nactive++;
+        }
+    }
+    if (nactive == 0) return 0;
+    h_skip.assign(nmol, 0);
+    for (int m = 0; m < nmol; m++) h_skip[m] = !run_active[m];
+    B_CHECK(hipMemcpyAsync(d_skip, h_skip.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, stream));
+    t0 = bwtime();
+    if (diis(iter, run_active)) return -1;
+    B_CHECK(hipStreamSynchronize(stream));
+    const double t1 = bwtime();
+    time_diis += t1 - t0;
+    if (purify(run_active)) return -1;
+    B_CHECK(hipStreamSynchronize(stream));
+    time_purif += bwtime() - t1;
+    return nactive;
+}
+
+// every molecule's quartets in later plain builds (gtf_batch_compute_fock)
+int GtfBatch::scf_finish()
+{
+    h_active.assign(nmol, 1);
+    B_CHECK(hipMemcpyAsync(sys.d_mol_active, h_active.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, stream));
+    B_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
 // compute_diis (pscf/purif.c:491-641) for the active molecules: F (packed,
-// AO) and D (packed) in, the orthogonal-basis extrapolated F in p_F
-int GtfBatch::diis(int iter, const std::vector<int>& active)
+// AO) and D (packed) in, the orthogonal-basis extrapolated F in p_F.
+// iter[m] is molecule m's own iteration index (molecules that joined a
+// running batch later count from their own start): the error vector and
+// extrapolation apply from iter[m] > 1 on, as scf.c's compute_diis.
+int GtfBatch::diis(const std::vector<int>& iter, const std::vector<int>& active)
 {
     const BatchLayout L = layout();
     const int LB = B_MAX_DIIS + 1;
     const int* skip = d_skip;
     if (gtf_bpad(stream, L, skip, d_F, p_W2) || gtf_bpad(stream, L, skip, d_D, p_D)) return -1;
-    std::vector<int> slot(nmol, 0), nvec(nmol, 0);
-    if (iter > 1) {
-        for (int m = 0; m < nmol; m++) {
-            if (!active[m]) continue;
+    std::vector<int> slot(nmol, -1), nvec(nmol, 0), skip_x(nmol, 1);
+    bool any = false;
+    for (int m = 0; m < nmol; m++)
+        if (active[m] && iter[m] > 1) {
+            any = true;
+            skip_x[m] = 0;
             slot[m] = len_diis[m] < B_MAX_DIIS ? len_diis[m]++ : bmax_id[m];
             if (slot[m] < 0 || slot[m] >= B_MAX_DIIS) {
                 // only a non-finite B matrix leaves no largest error; such a
@@ -360,22 +492,30 @@ int GtfBatch::diis(int iter, const std::vector<int>& active)
             }
             nvec[m] = len_diis[m];
         }
-        B_CHECK(hipMemcpyAsync(d_slot, slot.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, stream));
-        B_CHECK(hipMemcpyAsync(d_nvec, nvec.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, stream));
+    // molecules without extrapolation this iteration: skipped by the slot
+    // stores and the linear combination
+    const int* skip2 = d_skip2;
+    if (any) {
+        h_skip2 = skip_x;
+        h_slot = slot;
+        h_nvec = nvec;
+        B_CHECK(hipMemcpyAsync(d_skip2, h_skip2.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, stream));
+        B_CHECK(hipMemcpyAsync(d_slot, h_slot.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, stream));
+        B_CHECK(hipMemcpyAsync(d_nvec, h_nvec.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, stream));
         // e = X (F D S - S D F) X with the A * B^T convention of pdgemm3D_2
-        if (gtf_bgemm(stream, L, skip, nullptr, p_W2, p_D, p_W, 1) ||
-            gtf_bgemm(stream, L, skip, nullptr, p_W, p_S, p_D2, 1) || gtf_bantisym(stream, L, skip, p_D2, p_D3) ||
-            gtf_bgemm(stream, L, skip, nullptr, p_X, p_D3, p_W, 1) ||
-            gtf_bgemm(stream, L, skip, nullptr, p_W, p_X, p_D2, 1))
+        if (gtf_bgemm(stream, L, skip2, nullptr, p_W2, p_D, p_W, 1) ||
+            gtf_bgemm(stream, L, skip2, nullptr, p_W, p_S, p_D2, 1) || gtf_bantisym(stream, L, skip2, p_D2, p_D3) ||
+            gtf_bgemm(stream, L, skip2, nullptr, p_X, p_D3, p_W, 1) ||
+            gtf_bgemm(stream, L, skip2, nullptr, p_W, p_X, p_D2, 1))
             return -1;
-        bstore_slot_kernel<<<dim3(256, nmol), 256, 0, stream>>>(d_poff, d_pld, skip, d_slot, p_D2, evecs, ptotal);
+        bstore_slot_kernel<<<dim3(256, nmol), 256, 0, stream>>>(d_poff, d_pld, skip2, d_slot, p_D2, evecs, ptotal);
         B_CHECK(hipGetLastError());
-        if (gtf_bmultidot(stream, L, skip, d_nvec, p_D2, evecs, ptotal, d_dots)) return -1;
+        if (gtf_bmultidot(stream, L, skip2, d_nvec, p_D2, evecs, ptotal, d_dots)) return -1;
         std::vector<double> dots((size_t)BMAX_DOTS * nmol);
         B_CHECK(hipMemcpyAsync(dots.data(), d_dots, sizeof(double) * dots.size(), hipMemcpyDeviceToHost, stream));
         B_CHECK(hipStreamSynchronize(stream));
         for (int m = 0; m < nmol; m++) {
-            if (!active[m]) continue;
+            if (skip_x[m]) continue;
             double* bm = &b_mat[(size_t)m * LB * LB];
             const int cur = slot[m];
             for (int i = 0; i < len_diis[m]; i++) {
@@ -394,19 +534,19 @@ int GtfBatch::diis(int iter, const std::vector<int>& active)
     if (gtf_bgemm(stream, L, skip, nullptr, p_X, p_W2, p_W, 1) ||
         gtf_bgemm(stream, L, skip, nullptr, p_W, p_X, p_F, 1))
         return -1;
-    if (iter > 1) {
-        bstore_slot_kernel<<<dim3(256, nmol), 256, 0, stream>>>(d_poff, d_pld, skip, d_slot, p_F, Fvecs, ptotal);
+    if (any) {
+        bstore_slot_kernel<<<dim3(256, nmol), 256, 0, stream>>>(d_poff, d_pld, skip2, d_slot, p_F, Fvecs, ptotal);
         B_CHECK(hipGetLastError());
-        std::vector<double> coef((size_t)BMAX_DOTS * nmol, 0.0);
+        h_coef.assign((size_t)BMAX_DOTS * nmol, 0.0);
         for (int m = 0; m < nmol; m++) {
-            if (!active[m]) continue;
+            if (skip_x[m]) continue;
             const int sz = len_diis[m] + 1;
             double inv[(B_MAX_DIIS + 1) * (B_MAX_DIIS + 1)];
             binverse(sz, &b_mat[(size_t)m * LB * LB], LB, inv);
-            for (int i = 0; i < len_diis[m]; i++) coef[(size_t)m * BMAX_DOTS + i] = -inv[i * sz + sz - 1];
+            for (int i = 0; i < len_diis[m]; i++) h_coef[(size_t)m * BMAX_DOTS + i] = -inv[i * sz + sz - 1];
         }
-        B_CHECK(hipMemcpyAsync(d_coef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, stream));
-        if (gtf_blincomb(stream, L, skip, d_nvec, d_coef, Fvecs, ptotal, p_F)) return -1;
+        B_CHECK(hipMemcpyAsync(d_coef, h_coef.data(), sizeof(double) * h_coef.size(), hipMemcpyHostToDevice, stream));
+        if (gtf_blincomb(stream, L, skip2, d_nvec, d_coef, Fvecs, ptotal, p_F)) return -1;
     }
     return 0;
 }
@@ -419,16 +559,30 @@ int GtfBatch::purify(const std::vector<int>& active)
     const int* skip = d_skip;
     if (gtf_bpurif_init(stream, L, d_nocc, skip, p_F, p_D)) return -1;
     if (gtf_fill_int(stream, d_iters, nmol, -1)) return -1;
-    if (gtf_fill_int(stream, d_fp32, nmol, purif_fp32 ? 1 : 0)) return -1;
+    if (gtf_fill_int(stream, d_low, nmol, purif_mode)) return -1;
+    if (purif_mode == GTF_LOW_SPLIT && !oz_mem) {
+        oz_mem = sys.dalloc<char>(gtf_oz_scratch_bytes(ptotal, OZ_SLICES_MAX));
+        if (!oz_mem) return -2;
+        gtf_oz_scratch_carve(oz_mem, ptotal, purif_ns, oz);
+    }
+    OzOperands o;
+    o.nmol = nmol;
+    o.maxld = maxpld;
+    o.poff = d_poff;
+    o.pld = d_pld;
+    o.skip = skip;
+    o.done = d_iters;
+    o.sel = d_low;
     std::vector<int> it_h(nmol, -1);
     int it = 0;
     for (it = 0; it < 200; it++) {
-        // molecules already converged skip their GEMMs through d_iters >= 0
-        if (gtf_bgemm(stream, L, skip, d_iters, p_D, p_D, p_D2, 0, d_fp32) ||
-            gtf_bgemm(stream, L, skip, d_iters, p_D, p_D2, p_D3, 0, d_fp32))
+        // molecules already converged skip their GEMMs through d_iters >= 0;
+        // those in the split-BF16 phase take the ozaki.h products instead
+        if (gtf_bgemm(stream, L, skip, d_iters, p_D, p_D, p_D2, 0, d_low) ||
+            gtf_bgemm(stream, L, skip, d_iters, p_D, p_D2, p_D3, 0, d_low))
             return -1;
-        if (gtf_bpurif_stats(stream, L, skip, p_D, p_D2, p_D3, d_iters, d_stats, it, 1e-11, d_fp32,
-                             purif_fp32_until))
+        if (purif_mode == GTF_LOW_SPLIT && gtf_oz_square_cube(stream, o, oz, p_D, p_D2, p_D3)) return -1;
+        if (gtf_bpurif_stats(stream, L, skip, p_D, p_D2, p_D3, d_iters, d_stats, it, 1e-11, d_low, purif_until))
             return -1;
         if (gtf_bpurif_update(stream, L, skip, d_iters, d_stats, p_D, p_D2, p_D3, it)) return -1;
         if (it % 2 == 1) {
@@ -491,6 +645,7 @@ int gtf_batch_create(BasisSet_t* basis, int nmol, double tolscr, gtf_batch_t* ou
     }
     int rc = b->sys.init(b->bases, b->stream);
     if (rc == 0) rc = b->sys.screen(tolscr);
+    b->tolscr = tolscr;
     if (rc) { delete b; return rc; }
     GtfSystem& s = b->sys;
     b->d_off = s.dalloc<long>(nmol);
@@ -587,11 +742,19 @@ int gtf_batch_set_tile_shape(gtf_batch_t b, int cls, const int* shape)
     return b->sys.set_shape(cls, shape);
 }
 
-int gtf_batch_set_purif_precision(gtf_batch_t b, int fp32, double fp32_until)
+int gtf_batch_set_purif_precision(gtf_batch_t b, int mode, double until)
 {
-    if (!b || fp32_until <= 0.0) return -3;
-    b->purif_fp32 = fp32 != 0;
-    b->purif_fp32_until = fp32_until;
+    if (!b || until <= 0.0 || mode < 0 || mode > GTF_LOW_SPLIT) return -3;
+    b->purif_mode = mode;
+    b->purif_until = until;
+    return 0;
+}
+
+int gtf_batch_set_purif_slices(gtf_batch_t b, int ns)
+{
+    if (!b || ns < OZ_SLICES_MIN || ns > OZ_SLICES_MAX) return -3;
+    b->purif_ns = ns;
+    if (b->oz_mem) gtf_oz_scratch_carve(b->oz_mem, b->ptotal, ns, b->oz);
     return 0;
 }
 
@@ -685,72 +848,115 @@ int gtf_batch_scf_cb(gtf_batch_t b, int max_iters, double ediff, double* energy,
                      gtf_scf_done_fn done, void* ctx)
 {
     if (!b || max_iters <= 0 || !energy) return -3;
-    if (b->scf_setup()) return -1;
-    const int nmol = b->nmol;
-    const int LB = B_MAX_DIIS + 1;
-    b->len_diis.assign(nmol, 0);
-    b->bmax_id.assign(nmol, -1);
-    b->bmax.assign(nmol, DBL_MIN);
-    b->b_mat.assign((size_t)nmol * LB * LB, 0.0);
-    for (int m = 0; m < nmol; m++)
-        for (int i = 0; i <= B_MAX_DIIS; i++)
-            for (int j = 0; j <= B_MAX_DIIS; j++) b->b_mat[(size_t)m * LB * LB + i * LB + j] = (i != j) ? -1.0 : 0.0;
-    b->time_fock = b->time_diis = b->time_purif = 0.0;
-    if (gtf_fill(b->stream, b->d_D, b->total, 0.0)) return -1;
-    if (b->put_initial_guess()) return -1;
-    std::vector<int> active(nmol, 1), skip(nmol, 0), nit(nmol, max_iters);
-    std::vector<double> e0(nmol, -1.0), e(nmol, 0.0);
-    const BatchLayout L = b->layout();
-    for (int it = 0; it < max_iters; it++) {
-        double t0 = bwtime();
-        if (b->fock()) return -1;
-        if (gtf_bfock_energy(b->stream, L, b->d_H, b->d_G, b->d_D, b->d_F, b->d_energy)) return -1;
-        B_CHECK(hipMemcpyAsync(e.data(), b->d_energy, sizeof(double) * nmol, hipMemcpyDeviceToHost, b->stream));
-        B_CHECK(hipStreamSynchronize(b->stream));
-        b->time_fock += bwtime() - t0;
-        int nactive = 0;
-        for (int m = 0; m < nmol; m++) {
-            if (!active[m]) continue;
-            energy[m] = e[m] + b->enuc[m];
-            if (history) history[(size_t)m * max_iters + it] = energy[m];
-            if (!std::isfinite(e[m])) {
-                // diverged: stop it, report iters = -(builds)
-                active[m] = 0;
-                nit[m] = -(it + 1);
-                b->done_time[m] = bwtime();
-                if (done) done(ctx, m, energy[m], nit[m]);
-                continue;
-            }
-            if (it > 0 && fabs(e[m] - e0[m]) < ediff) {
-                active[m] = 0;
-                nit[m] = it + 1;
-                b->done_time[m] = bwtime();
-                if (done) done(ctx, m, energy[m], nit[m]);
-            } else {
-                e0[m] = e[m];
-                nactive++;
-            }
-        }
-        if (nactive == 0 || it == max_iters - 1) break;
-        for (int m = 0; m < nmol; m++) skip[m] = !active[m];
-        B_CHECK(hipMemcpyAsync(b->d_skip, skip.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, b->stream));
-        t0 = bwtime();
-        if (b->diis(it, active)) return -1;
-        B_CHECK(hipStreamSynchronize(b->stream));
-        const double t1 = bwtime();
-        b->time_diis += t1 - t0;
-        if (b->purify(active)) return -1;
-        B_CHECK(hipStreamSynchronize(b->stream));
-        b->time_purif += bwtime() - t1;
+    int rc = b->scf_begin(max_iters, ediff, done, ctx);
+    while (rc == 0) {
+        const int left = b->scf_step();
+        if (left < 0) rc = left;
+        if (left <= 0) break;
     }
-    const double tend = bwtime();
-    for (int m = 0; m < nmol; m++)
-        if (active[m]) {
-            b->done_time[m] = tend;
-            if (done) done(ctx, m, energy[m], nit[m]);
+    if (rc == 0) rc = b->scf_finish();
+    if (rc) return rc;
+    return gtf_batch_scf_results(b, energy, iters, history);
+}
+
+int gtf_batch_scf_begin(gtf_batch_t b, int max_iters, double ediff, gtf_scf_done_fn done, void* ctx)
+{
+    if (!b || max_iters <= 0) return -3;
+    return b->scf_begin(max_iters, ediff, done, ctx);
+}
+
+int gtf_batch_scf_step(gtf_batch_t b)
+{
+    if (!b || b->run_max_iters <= 0) return -3;
+    const int left = b->scf_step();
+    if (left == 0 && b->scf_finish()) return -1;
+    return left;
+}
+
+int gtf_batch_scf_results(gtf_batch_t b, double* energy, int* iters, double* history)
+{
+    if (!b || b->run_max_iters <= 0) return -3;
+    const int nmol = b->nmol, mi = b->run_max_iters;
+    for (int m = 0; m < nmol; m++) {
+        if (energy) energy[m] = b->run_energy[m];
+        if (iters) iters[m] = b->run_active[m] ? 0 : b->run_nit[m];
+        if (history)
+            for (int k = 0; k < mi; k++) history[(size_t)m * mi + k] = b->run_hist[(size_t)m * mi + k];
+    }
+    return nmol;
+}
+
+int gtf_batch_regroup(gtf_batch_t b, BasisSet_t* add, int nadd, gtf_batch_t* out, int* origin)
+{
+    if (!b || !out || nadd < 0 || (nadd > 0 && !add) || b->run_max_iters <= 0) return -3;
+    *out = nullptr;
+    std::vector<int> keep;
+    for (int m = 0; m < b->nmol; m++)
+        if (b->run_active[m]) keep.push_back(m);
+    const int nkeep = (int)keep.size(), nnew = nkeep + nadd;
+    if (nnew == 0) return -3;
+    std::vector<BasisSet_t> bases;
+    for (int m : keep) bases.push_back(const_cast<BasisSet_t>(b->bases[m]));
+    for (int j = 0; j < nadd; j++) bases.push_back(add[j]);
+    GtfBatch* nb = nullptr;
+    int rc = gtf_batch_create(bases.data(), nnew, b->tolscr, &nb);
+    if (rc) return rc;
+    // the settings of the running batch: tile shapes, precision, graph use
+    std::vector<int> shapes((size_t)GTF_NCLASSES * GTF_TILE_SHAPE_FIELDS);
+    for (int c = 0; c < GTF_NCLASSES && rc == 0; c++)
+        if (b->sys.get_shape(c, &shapes[(size_t)c * GTF_TILE_SHAPE_FIELDS]) < 0) rc = -1;
+    nb->sys.tau = b->sys.tau;  // takes effect with the shapes' tile rebuild
+    if (rc == 0) rc = nb->sys.set_shapes(shapes.data());
+    nb->use_graph = b->use_graph;
+    nb->purif_mode = b->purif_mode;
+    nb->purif_until = b->purif_until;
+    nb->purif_ns = b->purif_ns;
+    if (rc == 0) rc = nb->scf_begin(b->run_max_iters, b->run_ediff, b->run_done, b->run_ctx);
+    if (rc) {
+        delete nb;
+        return rc < 0 ? rc : -1;
+    }
+    // carry the running molecules' SCF state over exactly
+    const int LB = B_MAX_DIIS + 1, mi = b->run_max_iters;
+    for (int k = 0; k < nkeep; k++) {
+        const int mo = keep[k];
+        const size_t nn = (size_t)b->n[mo] * b->n[mo], pp = (size_t)b->pld[mo] * b->pld[mo];
+        bool ok = hipMemcpyAsync(nb->d_D + nb->off[k], b->d_D + b->off[mo], sizeof(double) * nn,
+                                 hipMemcpyDeviceToDevice, nb->stream) == hipSuccess;
+        for (int sl = 0; sl < B_MAX_DIIS && ok; sl++) {
+            ok = hipMemcpyAsync(nb->evecs + (size_t)sl * nb->ptotal + nb->poff[k],
+                                b->evecs + (size_t)sl * b->ptotal + b->poff[mo], sizeof(double) * pp,
+                                hipMemcpyDeviceToDevice, nb->stream) == hipSuccess &&
+                 hipMemcpyAsync(nb->Fvecs + (size_t)sl * nb->ptotal + nb->poff[k],
+                                b->Fvecs + (size_t)sl * b->ptotal + b->poff[mo], sizeof(double) * pp,
+                                hipMemcpyDeviceToDevice, nb->stream) == hipSuccess;
         }
-    if (iters)
-        for (int m = 0; m < nmol; m++) iters[m] = nit[m];
+        if (!ok) {
+            delete nb;
+            return -1;
+        }
+        nb->len_diis[k] = b->len_diis[mo];
+        nb->bmax_id[k] = b->bmax_id[mo];
+        nb->bmax[k] = b->bmax[mo];
+        std::copy(b->b_mat.begin() + (size_t)mo * LB * LB, b->b_mat.begin() + (size_t)(mo + 1) * LB * LB,
+                  nb->b_mat.begin() + (size_t)k * LB * LB);
+        nb->run_next[k] = b->run_next[mo];
+        nb->run_e0[k] = b->run_e0[mo];
+        nb->run_energy[k] = b->run_energy[mo];
+        nb->run_nit[k] = b->run_nit[mo];
+        std::copy(b->run_hist.begin() + (size_t)mo * mi, b->run_hist.begin() + (size_t)(mo + 1) * mi,
+                  nb->run_hist.begin() + (size_t)k * mi);
+        nb->purif_iters[k] = b->purif_iters[mo];
+    }
+    if (hipStreamSynchronize(nb->stream) != hipSuccess) {
+        delete nb;
+        return -1;
+    }
+    if (origin) {
+        for (int k = 0; k < nkeep; k++) origin[k] = keep[k];
+        for (int j = 0; j < nadd; j++) origin[nkeep + j] = -(j + 1);
+    }
+    *out = nb;
     return 0;
 }
 

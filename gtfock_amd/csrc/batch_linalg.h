@@ -23,13 +23,14 @@ struct BatchLayout {
 };
 
 // skip[m] != 0 or done[m] >= 0 (either array may be null) leaves molecule m
-// alone; fp32[m] != 0 (fp32 may be null) multiplies its operands in FP32
+// alone; fp32[m] (fp32 may be null) == GTF_LOW_FP32 multiplies its operands
+// in FP32, == GTF_LOW_SPLIT leaves it to the split-BF16 products
 int gtf_bgemm(hipStream_t st, const BatchLayout& L, const int* skip, const int* done, const double* A,
               const double* B, double* C, int transB, const int* fp32 = nullptr);
 int gtf_bpad(hipStream_t st, const BatchLayout& L, const int* skip, const double* packed, double* padded);
 int gtf_bunpad(hipStream_t st, const BatchLayout& L, const int* skip, const double* padded, double* packed);
 int gtf_bpurif_init(hipStream_t st, const BatchLayout& L, const int* nocc, const int* skip, const double* F, double* D);
-// also clears fp32[m] (may be null) once ||D - D^2||_F < fp32_until
+// also sets fp32[m] (may be null) to GTF_LOW_FP64 once ||D - D^2||_F < fp32_until
 int gtf_bpurif_stats(hipStream_t st, const BatchLayout& L, const int* skip, const double* D, const double* D2,
                      const double* D3, int* iters, double* stats, int it, double tol, int* fp32 = nullptr,
                      double fp32_until = 0.0);

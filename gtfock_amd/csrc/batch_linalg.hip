@@ -12,11 +12,13 @@
 #include "batch_linalg.h"
 
 #include "gemm_tile.h"
+#include "ozaki_launch.h"
 
 // C_m = A_m * B_m (TRANSB = 0) or A_m * B_m^T (TRANSB = 1) on padded blocks
 // (gemm_tile.h).  A molecule is skipped when skip[m] != 0 or, during
-// purification, when it has already converged (done[m] >= 0); fp32[m] != 0
-// selects the FP32 MFMA tile for it.
+// purification, when it has already converged (done[m] >= 0); fp32[m] ==
+// GTF_LOW_FP32 selects the FP32 MFMA tile for it, GTF_LOW_SPLIT leaves it to
+// the split-BF16 products.
 template <int TRANSB>
 __global__ __launch_bounds__(256) void bgemm_kernel(const long* __restrict__ poff, const int* __restrict__ pld,
                                                     const int* __restrict__ skip, const int* __restrict__ done,
@@ -28,7 +30,8 @@ __global__ __launch_bounds__(256) void bgemm_kernel(const long* __restrict__ pof
     const int row0 = blockIdx.y * GEMM_GT, col0 = blockIdx.x * GEMM_GT;
     if (row0 >= ld || col0 >= ld || (skip && skip[m]) || (done && done[m] >= 0)) return;
     const long o = poff[m];
-    if (fp32 && fp32[m]) gemm_tile<TRANSB, true>(ld, A0 + o, B0 + o, C0 + o, row0, col0);
+    if (fp32 && fp32[m] == GTF_LOW_SPLIT) return;  // the split-BF16 products (ozaki.hip) cover it
+    if (fp32 && fp32[m] == GTF_LOW_FP32) gemm_tile<TRANSB, true>(ld, A0 + o, B0 + o, C0 + o, row0, col0);
     else gemm_tile<TRANSB, false>(ld, A0 + o, B0 + o, C0 + o, row0, col0);
 }
 

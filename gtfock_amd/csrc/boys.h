@@ -10,62 +10,49 @@
 //   F_m(T) = exp(-T) sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1))
 // in long double (gtf_boys_table in basis.cpp).
 #pragma once
-#include <type_traits>
-
 #include "gtf_internal.h"
 
 // F_0..F_M(T) for the ERI kernels from the compact Taylor table: slab holds,
 // for grid points T0 = i * BOYS8_DELTA (i = 0 .. BOYS8_NGRID - 1), the
-// BOYS8_NTERM coefficients c_k = F_{M+k}(T0) / k! of order M, one 64-byte
-// (FP64) or 32-byte (FP32) row per point, so that
-//   F_M(T) = sum_k c_k (T0 - T)^k,  |T0 - T| <= BOYS8_DELTA / 2.
-// FP64 uses all 8 terms (truncation below 1e-15 relative), FP32 the first 6
-// (below 3e-11 relative).  A class touches one slab: 23 KB (FP64), 12 KB
-// (FP32), small enough to stay in the vector L1 cache.
-template <int M, typename Real>
-__device__ __forceinline__ void boys_eval8(Real T, Real* F, const Real* __restrict__ slab)
+// BOYS8_NTERM = 8 coefficients c_k = F_{M+k}(T0) / k! of order M, one 64-byte
+// row per point, so that
+//   F_M(T) = sum_k c_k (T0 - T)^k,  |T0 - T| <= BOYS8_DELTA / 2
+// (truncation below 1e-15 relative).  FP64 in every kernel, the FP32 tiles
+// included.  A class touches one slab (23 KB), small enough to stay in the
+// vector L1 cache or in LDS.
+template <int M>
+__device__ __forceinline__ void boys_eval8(double T, double* F, const double* __restrict__ slab)
 {
-    if (T < Real(BOYS_TMAX)) {
-        const int i = (int)(T * Real(BOYS8_INV_DELTA) + Real(0.5));
-        const Real dt = Real(i) * Real(BOYS8_DELTA) - T;
-        Real f;
-        if constexpr (std::is_same<Real, double>::value) {
-            const double2* r = reinterpret_cast<const double2*>(slab + i * BOYS8_NTERM);
-            const double2 c01 = r[0], c23 = r[1], c45 = r[2], c67 = r[3];
-            f = fma(c67.y, dt, c67.x);
-            f = fma(f, dt, c45.y);
-            f = fma(f, dt, c45.x);
-            f = fma(f, dt, c23.y);
-            f = fma(f, dt, c23.x);
-            f = fma(f, dt, c01.y);
-            f = fma(f, dt, c01.x);
-        } else {
-            const float4* r = reinterpret_cast<const float4*>(slab + i * BOYS8_NTERM);
-            const float4 c03 = r[0], c47 = r[1];
-            f = fmaf(c47.y, dt, c47.x);
-            f = fmaf(f, dt, c03.w);
-            f = fmaf(f, dt, c03.z);
-            f = fmaf(f, dt, c03.y);
-            f = fmaf(f, dt, c03.x);
-        }
+    if (T < BOYS_TMAX) {
+        const int i = (int)(T * BOYS8_INV_DELTA + 0.5);
+        const double dt = i * BOYS8_DELTA - T;
+        const double2* r = reinterpret_cast<const double2*>(slab + i * BOYS8_NTERM);
+        const double2 c01 = r[0], c23 = r[1], c45 = r[2], c67 = r[3];
+        double f = fma(c67.y, dt, c67.x);
+        f = fma(f, dt, c45.y);
+        f = fma(f, dt, c45.x);
+        f = fma(f, dt, c23.y);
+        f = fma(f, dt, c23.x);
+        f = fma(f, dt, c01.y);
+        f = fma(f, dt, c01.x);
         F[M] = f;
         if constexpr (M > 0) {
-            const Real e = exp(-T);
-            const Real t2 = T + T;
+            const double e = exp(-T);
+            const double t2 = T + T;
             static_for<0, M>([&](auto k) {
                 constexpr int m = M - 1 - decltype(k)::value;
-                F[m] = (t2 * F[m + 1] + e) * Real(1.0 / (2 * m + 1));
+                F[m] = (t2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
             });
         }
     } else {
-        const Real rt = rsqrt_nr(T);
-        F[0] = Real(0.88622692545275801365) * rt;  // sqrt(pi)/2
+        const double rt = rsqrt_nr(T);
+        F[0] = 0.88622692545275801365 * rt;  // sqrt(pi)/2
         if constexpr (M > 0) {
-            const Real e = exp(-T);
-            const Real h = Real(0.5) * rt * rt;
+            const double e = exp(-T);
+            const double h = 0.5 * rt * rt;
             static_for<1, M + 1>([&](auto k) {
                 constexpr int m = decltype(k)::value;
-                F[m] = (Real(2 * m - 1) * F[m - 1] - e) * h;
+                F[m] = ((2 * m - 1) * F[m - 1] - e) * h;
             });
         }
     }

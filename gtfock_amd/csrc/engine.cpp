@@ -93,6 +93,8 @@ int GtfSystem::init(const std::vector<const BasisSet*>& mols, hipStream_t st)
     d_mld = dalloc<int>(nshells);
     d_mol_off = dalloc<long>(nmol);
     d_mol_n = dalloc<int>(nmol);
+    d_shell_mol = dalloc<int>(nshells);
+    d_mol_active = dalloc<int>(nmol);
     d_alpha = dalloc<double>(alpha.size());
     d_coef = dalloc<double>(coef.size());
     d_center = dalloc<double>(3 * nshells);
@@ -100,8 +102,10 @@ int GtfSystem::init(const std::vector<const BasisSet*>& mols, hipStream_t st)
     d_atom_Z = dalloc<double>(natoms);
     d_boys = dalloc<double>((size_t)BOYS_NGRID * BOYS_NCOL);
     if (!d_L || !d_nprim || !d_poff || !d_fstart || !d_atom || !d_mbase || !d_mld || !d_mol_off || !d_mol_n ||
-        !d_alpha || !d_coef || !d_center || !d_atom_xyz || !d_atom_Z || !d_boys)
+        !d_alpha || !d_coef || !d_center || !d_atom_xyz || !d_atom_Z || !d_boys || !d_shell_mol || !d_mol_active)
         return -2;
+    const std::vector<int> all_active(nmol, 1);
+    if (upload(d_shell_mol, h_shell_mol, st) || upload(d_mol_active, all_active, st)) return -1;
     if (upload(d_L, L, st) || upload(d_nprim, nprim, st) || upload(d_poff, poff, st) ||
         upload(d_fstart, fstart, st) || upload(d_atom, atom, st) || upload(d_mbase, mbase, st) ||
         upload(d_mld, mld, st) || upload(d_mol_off, mol_mat_off, st) || upload(d_mol_n, moln, st) ||
@@ -112,12 +116,10 @@ int GtfSystem::init(const std::vector<const BasisSet*>& mols, hipStream_t st)
     gtf_boys_table(boys.data(), BOYS_NGRID, BOYS_NCOL, BOYS_DELTA);
     if (upload(d_boys, boys, st)) return -1;
     std::vector<double> b8((size_t)BOYS8_NM * BOYS8_SLAB);
-    std::vector<float> b8f(b8.size());
-    gtf_boys_taylor_tables(b8.data(), b8f.data());
+    gtf_boys_taylor_tables(b8.data());
     d_boys8 = dalloc<double>(b8.size());
-    d_boys8f = dalloc<float>(b8f.size());
-    if (!d_boys8 || !d_boys8f) return -2;
-    if (upload(d_boys8, b8, st) || upload(d_boys8f, b8f, st)) return -1;
+    if (!d_boys8) return -2;
+    if (upload(d_boys8, b8, st)) return -1;
     // the uploads read host vectors that go out of scope here
     SYS_CHECK(hipStreamSynchronize(st));
 
@@ -245,8 +247,7 @@ int GtfSystem::screen(double tolscr)
             a.pv = pv;
             a.fstart = d_fstart;
             a.boys8 = d_boys8;
-            a.boys8f = d_boys8f;
-            a.vbits = d_vbits;
+                    a.vbits = d_vbits;
             if (launch(ci->diag, a, (int)((a.nunits + 255) / 256), stream)) return -1;
             off += n;
         }
@@ -673,12 +674,13 @@ int GtfSystem::launch_class(int cls, hipStream_t cs, const double* D, double* J,
     a.pv = pv;
     a.fstart = d_fstart;
     a.boys8 = d_boys8;
-    a.boys8f = d_boys8f;
     a.D = D;
     a.J = J;
     a.K = K;
     a.mbase = d_mbase;
     a.mld = d_mld;
+    a.shell_mol = d_shell_mol;
+    a.mol_active = d_mol_active;
     a.row_bra = d_row_bra[cls];
     a.row_kstart = d_row_kstart[cls];
     if (ntiles[cls] > 0 || ntiles_lo[cls] > 0) {
@@ -938,7 +940,6 @@ int GtfSystem::store(int cls, int nq_, const int2* d_qlist, double* d_out)
     a.pv = pair_view();
     a.fstart = d_fstart;
     a.boys8 = d_boys8;
-    a.boys8f = d_boys8f;
     a.out = d_out;
     return launch(ci->store, a, (int)((a.nunits + 255) / 256), stream);
 }

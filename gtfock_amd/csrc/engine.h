@@ -38,9 +38,10 @@ struct GtfSystem {
     int* d_mld = nullptr;     // per shell: leading dimension (n of its molecule)
     long* d_mol_off = nullptr;
     int* d_mol_n = nullptr;
+    int* d_shell_mol = nullptr;   // per shell: its molecule
+    int* d_mol_active = nullptr;  // per molecule: 1 = its J/K are built, 0 = the ERI kernels skip its quartets
     double* d_boys = nullptr;     // one-electron grid (BOYS_NGRID x BOYS_NCOL)
     double* d_boys8 = nullptr;    // ERI Taylor tables, FP64
-    float* d_boys8f = nullptr;    // and FP32
     // ---- shell pairs of every molecule, canonical orientation L(s1) >= L(s2)
     int npairs = 0;
     std::vector<int> h_s1, h_s2, h_type, h_pnprim;

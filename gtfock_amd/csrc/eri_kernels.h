@@ -32,7 +32,6 @@ struct EriArgs {
     PairView pv;
     const int* fstart;
     const double* boys8;  // compact Taylor tables, BOYS8_NM slabs of BOYS8_SLAB
-    const float* boys8f;
     // EPI_JK
     const double* D;
     double* J;
@@ -61,7 +60,17 @@ struct EriArgs {
     // ket of each row
     const int* row_bra;
     const int* row_kstart;
+    // batches: molecule of each shell and, per molecule, whether its J/K are
+    // built (0: its quartets are skipped; a molecule whose SCF has ended)
+    const int* shell_mol;
+    const int* mol_active;
 };
+
+// the bra pair's molecule takes part in this build
+__device__ __forceinline__ bool bra_active(const EriArgs& a, int bra)
+{
+    return !a.mol_active || a.mol_active[a.shell_mol[a.pv.s1[bra]]];
+}
 
 // the Schwarz test of pfock/fock_task.c:192 and, for a diagonal class, the
 // one-orientation rule, for bra row r and ket slot j
@@ -106,6 +115,9 @@ __device__ __forceinline__ void eri_unit(const EriArgs& a, long u)
         if (!quartet_valid(a, r, j)) return;
         bra = a.bra_list[r];
         ket = a.ket_list[j];
+    }
+    if constexpr (EPI == EPI_JK) {
+        if (!bra_active(a, bra)) return;
     }
     double acc[NC][NCD];
     eri_block<LA, LB, LC, LD, NC, double>(a.pv, bra, ket, c0, a.boys8 + (LA + LB + LC + LD) * BOYS8_SLAB, acc);
@@ -288,49 +300,36 @@ __global__ __launch_bounds__(256) void eri_kernel(EriArgs a)
 #define GTF_UNIT_PQ 64
 #define GTF_TILE_PQ 65536
 
-// The Boys Taylor slab of order L (FP64 or FP32) copied from HBM into the
-// first GTF_BOYS_LDS doubles of dynamic LDS.  A persistent workgroup copies
-// it once; every primitive quartet of every tile it takes then reads its
-// Taylor row from LDS instead of a dependent global load.  The barrier that
-// follows each tile's clearing of its LDS work area orders the copy before
-// the first read.  It costs 23 KB of LDS per workgroup, which lowers the
-// occupancy of classes whose strips are already large, so it is a per-class
-// choice (the tile shape's boys_lds).
+// The Boys Taylor slab of order L copied from HBM into the first
+// GTF_BOYS_LDS doubles of dynamic LDS.  A persistent workgroup copies it
+// once; every primitive quartet of every tile it takes then reads its Taylor
+// row from LDS instead of a dependent global load.  The barrier that follows
+// each tile's clearing of its LDS work area orders the copy before the first
+// read.  It costs 23 KB of LDS per workgroup, which lowers the occupancy of
+// classes whose strips are already large, so it is a per-class choice (the
+// tile shape's boys_lds).  FP32 tiles use the same FP64 slab: the Boys
+// function is FP64 in both precisions.
 #define GTF_BOYS_LDS BOYS8_SLAB
-template <int L, typename Real>
-__device__ __forceinline__ const Real* boys_to_lds(const EriArgs& a, double* smem)
+template <int L>
+__device__ __forceinline__ const double* boys_to_lds(const EriArgs& a, double* smem)
 {
-    Real* dst = reinterpret_cast<Real*>(smem);
-    const Real* src;
-    if constexpr (std::is_same<Real, float>::value)
-        src = a.boys8f + L * BOYS8_SLAB;
-    else
-        src = a.boys8 + L * BOYS8_SLAB;
-    for (int k = threadIdx.x; k < BOYS8_SLAB; k += 256) dst[k] = src[k];
-    return dst;
-}
-
-// the same slab read in place from HBM (through the caches)
-template <int L, typename Real>
-__device__ __forceinline__ const Real* boys_global(const EriArgs& a)
-{
-    if constexpr (std::is_same<Real, float>::value)
-        return a.boys8f + L * BOYS8_SLAB;
-    else
-        return a.boys8 + L * BOYS8_SLAB;
+    const double* src = a.boys8 + L * BOYS8_SLAB;
+    for (int k = threadIdx.x; k < BOYS8_SLAB; k += 256) smem[k] = src[k];
+    return smem;
 }
 
 // Boys slab of a tile kernel: in LDS (BLDS; the work area then starts after
-// it) or read from HBM (the work area starts at the LDS base)
-template <int L, typename Real, bool BLDS>
-__device__ __forceinline__ const Real* tile_boys(const EriArgs& a, double* smem, double*& work)
+// it) or read in place from HBM through the caches (the work area starts at
+// the LDS base)
+template <int L, bool BLDS>
+__device__ __forceinline__ const double* tile_boys(const EriArgs& a, double* smem, double*& work)
 {
     if constexpr (BLDS) {
         work = smem + GTF_BOYS_LDS;
-        return boys_to_lds<L, Real>(a, smem);
+        return boys_to_lds<L>(a, smem);
     } else {
         work = smem;
-        return boys_global<L, Real>(a);
+        return a.boys8 + L * BOYS8_SLAB;
     }
 }
 
@@ -357,7 +356,7 @@ __global__ __launch_bounds__(256, (tile_min_waves<LA, LB, LC, LD>())) void eri_j
     extern __shared__ double smem[];
     __shared__ int s_task;
     double* work;
-    const Real* boys = tile_boys<L, Real, BLDS>(a, smem, work);
+    const double* boys = tile_boys<L, BLDS>(a, smem, work);
     for (;;) {
         if (threadIdx.x == 0) s_task = atomicAdd(a.counter, 1);
         __syncthreads();
@@ -365,6 +364,10 @@ __global__ __launch_bounds__(256, (tile_min_waves<LA, LB, LC, LD>())) void eri_j
         if (task >= a.ntiles) break;
         const int4 t = a.tiles[task];
         const int bra = a.bra_list[t.x];
+        if (!bra_active(a, bra)) {
+            __syncthreads();  // every thread has read s_task before it is rewritten
+            continue;
+        }
         const int nbp = a.pv.nprim[bra];
         const int nic = t.w;  // bra-primitive chunks per quartet
         // strip rows are as wide as the bra's molecule (at most a.nbf)
@@ -551,7 +554,7 @@ __global__ __launch_bounds__(256) void eri_jk_tile_t_kernel(EriArgs a)
     extern __shared__ double smem[];
     __shared__ int s_task;
     double* work;
-    const Real* boys = tile_boys<L, Real, BLDS>(a, smem, work);
+    const double* boys = tile_boys<L, BLDS>(a, smem, work);
     for (;;) {
         if (threadIdx.x == 0) s_task = atomicAdd(a.counter, 1);
         __syncthreads();
@@ -559,6 +562,10 @@ __global__ __launch_bounds__(256) void eri_jk_tile_t_kernel(EriArgs a)
         if (task >= a.ntiles) break;
         const int4 t = a.tiles[task];
         const int bra = a.bra_list[t.x];
+        if (!bra_active(a, bra)) {
+            __syncthreads();  // every thread has read s_task before it is rewritten
+            continue;
+        }
         const int nbf = a.mld[a.pv.s1[bra]];
         const int nstrip = (NA + NB) * nbf;
         double* strip = work;

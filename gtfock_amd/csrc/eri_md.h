@@ -93,7 +93,7 @@ __host__ __device__ constexpr int default_nc()
 // whole bra pair (NC == NAB, c0 == 0) the bra side is treated the same way.
 template <int LA, int LB, int LC, int LD, int NC, typename Real>
 __device__ __forceinline__ void eri_block(const PairView& pv, int bra, int ket, int c0,
-                                          const Real* __restrict__ boys_slab,
+                                          const double* __restrict__ boys_slab,
                                           Real (&acc)[NC][EriShape<LA, LB, LC, LD>::NCD], int i0 = 0,
                                           int i1 = -1)
 {
@@ -128,16 +128,22 @@ __device__ __forceinline__ void eri_block(const PairView& pv, int bra, int ket, 
         });
         for (int j = 0; j < nk; ++j) {
             const double4 K = pk[j];
-            const Real p = Real(B.x), q = Real(K.x);
-            const Real X = Real(B.y - K.y), Y = Real(B.z - K.z), Z = Real(B.w - K.w);
-            const Real r = rsqrt_nr(p + q);
-            const Real alpha = p * q * (r * r);
-            const Real T = alpha * (X * X + Y * Y + Z * Z);
+            // T = alpha |PQ|^2 and F_0..F_L(T) in FP64 for both precisions;
+            // an FP32 tile rounds them for the R recursion and contractions
+            const double Xd = B.y - K.y, Yd = B.z - K.z, Zd = B.w - K.w;
+            const double rd = rsqrt_nr(B.x + K.x);
+            const double alphad = B.x * K.x * (rd * rd);
+            const double Td = alphad * (Xd * Xd + Yd * Yd + Zd * Zd);
+            double Fd[L + 1];
+            boys_eval8<L>(Td, Fd, boys_slab);
             Real F[L + 1];
-            boys_eval8<L, Real>(T, F, boys_slab);
-            const Real pref = Real(GTF_TWO_PI_2_5) * r;
+            static_for<0, L + 1>([&](auto m_) {
+                constexpr int m = decltype(m_)::value;
+                F[m] = Real(Fd[m]);
+            });
+            const Real pref = Real(GTF_TWO_PI_2_5 * rd);
             Real R[HL];
-            hermite_R<L, Real>(R, F, pref, Real(-2) * alpha, X, Y, Z);
+            hermite_R<L, Real>(R, F, pref, Real(-2.0 * alphad), Real(Xd), Real(Yd), Real(Zd));
             Real V[NC][HCD];
             static_for<0, HCD>([&](auto h2_) {
                 constexpr int h2 = decltype(h2_)::value;

@@ -17,6 +17,7 @@
 #include "../../include/purif.h"
 #include "engine.h"
 #include "linalg.h"
+#include "ozaki_launch.h"
 #include "screen.h"
 
 struct purif_dev {
@@ -27,12 +28,15 @@ struct purif_dev {
     double *W = nullptr, *W2 = nullptr, *evecs = nullptr, *Fvecs = nullptr;
     double *scal = nullptr, *partial = nullptr, *coef = nullptr;
     bool xs_loaded = false;
-    // FP32 MFMA for the purification products while ||D - D^2||_F exceeds
-    // fp32_until, FP64 from there to the end (the FP64 iterations converge to
-    // the same projector)
-    bool fp32 = false;
-    double fp32_until = 1e-3;
-    int fp32_iters = 0;
+    // Reduced-precision purification products while ||D - D^2||_F exceeds
+    // low_until, FP64 from there to the end: mode 1 = FP32 MFMA on rounded
+    // operands, mode 2 = split-BF16 MFMA with ns exact slices (ozaki.h)
+    int mode = 0;
+    double low_until = 1e-3;
+    int ns = 5;
+    int low_iters = 0;
+    void* oz_mem = nullptr;
+    OzScratch oz;
 };
 
 static double wtime()
@@ -72,6 +76,21 @@ static int load_xs(purif_t* p)
     if (d->xs_loaded) return 0;
     if (h2d(d, d->X, p->X_block, p->ldx) || h2d(d, d->S, p->S_block, p->ldx)) return -1;
     d->xs_loaded = true;
+    return 0;
+}
+
+// D2 = D * D and D3 = D * D2 in split BF16 (ozaki.h), timed like gemm_pair
+static int split_square_cube(purif_dev* d, double* acc)
+{
+    PD_CHECK(hipEventRecord(d->e0, d->stream));
+    OzOperands o;
+    o.maxld = d->ld;
+    if (gtf_oz_square_cube(d->stream, o, d->oz, d->D, d->D2, d->D3)) return -1;
+    PD_CHECK(hipEventRecord(d->e1, d->stream));
+    PD_CHECK(hipEventSynchronize(d->e1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, d->e0, d->e1);
+    if (acc) *acc += 1e-3 * ms;
     return 0;
 }
 
@@ -155,7 +174,7 @@ void destroy_purif(purif_t* p)
     if (d) {
         if (d->stream) (void)hipStreamSynchronize(d->stream);
         double* bufs[] = {d->F, d->D, d->D2, d->D3, d->X, d->S, d->W, d->W2, d->evecs, d->Fvecs,
-                          d->scal, d->coef, d->partial};
+                          d->scal, d->coef, d->partial, (double*)d->oz_mem};
         for (double* b : bufs)
             if (b) (void)hipFree(b);
         if (d->e0) (void)hipEventDestroy(d->e0);
@@ -185,11 +204,15 @@ static int purify(purif_t* p, double* F_block, double* D_block, int* iters)
     const double lambda = std::min((double)nocc / (hmax - mu), (double)(n - nocc) / (mu - hmin));
     if (gtf_purif_init(d->stream, d->F, d->D, n, ld, -lambda / n, lambda * mu / n + (double)nocc / n)) return -1;
     int it;
-    bool low = d->fp32;
-    d->fp32_iters = 0;
+    bool low = d->mode != 0;
+    d->low_iters = 0;
     for (it = 0; it < 200; it++) {
-        if (gemm_pair(d, d->D, d->D, d->D2, 0, d->D, d->D2, d->D3, 0, &p->timedgemm, low ? 1 : 0)) return -1;
-        d->fp32_iters += low ? 1 : 0;
+        if (low && d->mode == 2) {
+            if (split_square_cube(d, &p->timedgemm)) return -1;
+        } else if (gemm_pair(d, d->D, d->D, d->D2, 0, d->D, d->D2, d->D3, 0, &p->timedgemm, low ? 1 : 0)) {
+            return -1;
+        }
+        d->low_iters += low ? 1 : 0;
         const double t0 = wtime();
         if (gtf_purif_stats(d->stream, d->D, d->D2, d->D3, n, ld, d->partial, d->scal)) return -1;
         double s[3];
@@ -198,7 +221,7 @@ static int purify(purif_t* p, double* F_block, double* D_block, int* iters)
         const double err = sqrt(s[0]), c = s[1] / s[2];
         if (gtf_purif_update(d->stream, d->D, d->D2, d->D3, m, c)) return -1;
         p->timetr += wtime() - t0;
-        if (low && err < d->fp32_until) low = false;
+        if (low && err < d->low_until) low = false;
         if (err < 1e-11) break;
     }
     // D = X D X (pscf/purif.c:469-476)
@@ -307,15 +330,33 @@ static int diis(purif_t* p, double* D_block, double* F_block, int iter)
     return d2h(d, F_block, curF, p->ldx);
 }
 
-int gtf_purif_set_precision(purif_t* p, int fp32, double fp32_until)
+int gtf_purif_set_precision(purif_t* p, int mode, double until)
 {
-    if (!p || !p->dev || fp32_until <= 0.0) return -1;
-    p->dev->fp32 = fp32 != 0;
-    p->dev->fp32_until = fp32_until;
+    if (!p || !p->dev || until <= 0.0 || mode < 0 || mode > 2) return -1;
+    purif_dev* d = p->dev;
+    if (mode == 2 && !d->oz_mem) {
+        const long total = (long)d->ld * d->ld;
+        if (hipMalloc(&d->oz_mem, gtf_oz_scratch_bytes(total, OZ_SLICES_MAX)) != hipSuccess) {
+            d->oz_mem = nullptr;
+            return -2;
+        }
+        gtf_oz_scratch_carve(d->oz_mem, total, d->ns, d->oz);
+    }
+    d->mode = mode;
+    d->low_until = until;
     return 0;
 }
 
-int gtf_purif_fp32_iters(purif_t* p) { return p && p->dev ? p->dev->fp32_iters : -1; }
+int gtf_purif_set_slices(purif_t* p, int ns)
+{
+    if (!p || !p->dev || ns < OZ_SLICES_MIN || ns > OZ_SLICES_MAX) return -1;
+    purif_dev* d = p->dev;
+    d->ns = ns;
+    if (d->oz_mem) gtf_oz_scratch_carve(d->oz_mem, (long)d->ld * d->ld, ns, d->oz);
+    return 0;
+}
+
+int gtf_purif_fp32_iters(purif_t* p) { return p && p->dev ? p->dev->low_iters : -1; }
 
 void compute_diis(PFock_t pfock, purif_t* p, double* D_block, double* F_block, int iter)
 {

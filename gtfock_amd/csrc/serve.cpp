@@ -1,19 +1,30 @@
-// serve.cpp -- an SCF request server over batches.
+// serve.cpp -- an SCF request server over batches, with continuous
+// batching at SCF-iteration granularity.
 //
-// Jobs (one molecule each) queue on submission.  A scheduler thread admits
-// them in arrival order into cohorts: the oldest queued job opens a cohort
-// and takes the queued jobs of its size class (basis functions in the same
+// Jobs (one molecule each) queue on submission.  A scheduler thread opens
+// cohorts in arrival order: the oldest queued job opens a cohort and takes
+// the queued jobs of its size class (basis functions in the same
 // power-of-two band, so a small molecule does not iterate in lockstep with a
 // large one), oldest first, up to a job cap and the device-memory budget
 // (per-job footprint estimated from the basis before anything is allocated).
-// A cohort runs as one batch on its own thread and HIP streams; up to
-// max_cohorts cohorts run at once, so requests that arrive while others are
-// on the GPU start as soon as a cohort slot and memory are free, and their
-// kernels overlap with the running cohorts' on the device.  A job's result is
-// published the moment its own SCF ends; the cohort's memory returns to the
-// budget when its last molecule is done.  Every per-molecule quantity of a
-// batch is computed independently of the other molecules in it, so a job's
-// result does not depend on its cohort.
+// A cohort runs as one batch on its own thread and HIP streams, one SCF
+// iteration at a time (gtf_batch_scf_step); up to max_cohorts cohorts run at
+// once and their kernels overlap on the device.
+//
+// Between two iterations a cohort admits queued jobs of its size class that
+// fit the budget: it regroups (gtf_batch_regroup) into a new batch that
+// carries its running molecules' SCF state over exactly and starts the new
+// ones, and drops the molecules that have finished, so their memory returns
+// to the budget then.  While a cohort of a small size class has room, the
+// scheduler leaves that class's queued jobs to it rather than opening another
+// cohort; large-molecule jobs open cohorts as before and join one when the
+// large slots are taken.  A cohort also regroups without newcomers when a
+// queued job waits for memory that its finished molecules still hold.
+// Regroups cost a batch setup, so a cohort regroups at most every join_gap
+// iterations and spends at most about a third of its time on them.  A job's
+// result is published the moment its own SCF ends.  Every per-molecule
+// quantity of a batch is computed independently of the other molecules in
+// it, so a job's result does not depend on its cohort or on when it joined.
 //
 // Only the library's public C API is used here (the batch entry points and
 // CInt_getNumFuncs), so the scheduler builds and can be tested on the host
@@ -55,12 +66,19 @@ struct GtfServer {
     int max_cohorts = 1, max_jobs = 64;
     double tolscr = 1e-11, tau = 0.0, ediff = 1e-8;
     int max_iters = 30;
+    // purification products of the cohorts (gtf_batch_set_purif_precision)
+    int purif_mode = 0;
+    double purif_until = 1e-3;
+    // iterations between two regroups of a cohort (< 0: cohorts never admit
+    // queued jobs while they run)
+    int join_gap = 1;
     std::mutex mu;
     std::condition_variable cv;
     std::deque<long> pending;
     std::vector<SJob> jobs;
     double mem_in_use = 0, mem_peak = 0;
     int running = 0, ncohorts = 0, peak_running = 0;
+    long joins = 0, regroups = 0;
     // cohorts of large molecules running at once, and their cap: half the
     // cohort slots, so smaller requests always find a slot
     int running_large = 0, max_large = 1;
@@ -69,6 +87,10 @@ struct GtfServer {
     std::vector<std::thread> workers;
     std::vector<int> finished;  // cohorts whose thread has ended, not yet joined
     std::vector<int> worker_cohort;
+    // per cohort id: size class and free job slots while it admits joiners
+    // (-1 when it does not)
+    std::vector<int> cohort_class, cohort_room;
+    std::vector<double> cohort_keep;  // estimates of its running jobs
     // tile shapes of new cohorts by molecule size: a cohort takes the last
     // range that holds its largest basis (none: the built-in defaults)
     struct ShapeRange {
@@ -78,6 +100,9 @@ struct GtfServer {
     std::vector<ShapeRange> shape_ranges;
     void schedule();
     void run_cohort(std::vector<long> ids, double mem, int cohort, bool large);
+    bool class_has_room(int cls, double mem_est) const;
+    bool memory_blocked() const;
+    void start_job(long id, int cohort, double t);
 };
 
 // molecules of at least 384 basis functions count as large
@@ -93,7 +118,8 @@ static int size_class(int nbf)
 
 struct CohortCtx {
     GtfServer* s;
-    const std::vector<long>* ids;
+    std::vector<long> ids;  // job id of each molecule of the cohort's current batch
+    double ended_mem = 0;   // estimates of jobs that ended in the current batch
 };
 
 // called on the cohort's thread when molecule m's SCF has ended
@@ -101,53 +127,194 @@ static void publish(void* ctx, int m, double energy, int builds)
 {
     CohortCtx* c = (CohortCtx*)ctx;
     std::lock_guard<std::mutex> lk(c->s->mu);
-    SJob& j = c->s->jobs[(*c->ids)[m]];
+    SJob& j = c->s->jobs[c->ids[m]];
     j.energy = energy;
     j.iters = builds;
     j.t_done = swtime();
     j.status = builds < 0 ? -5 : 2;
+    c->ended_mem += j.mem_est;
     c->s->cv.notify_all();
+}
+
+// a running cohort of size class cls could take a job of mem_est bytes at
+// its next iteration (free slots, and the budget holds both batches of the
+// regroup)
+bool GtfServer::class_has_room(int cls, double mem_est) const
+{
+    for (size_t c = 0; c < cohort_class.size(); c++)
+        if (cohort_class[c] == cls && cohort_room[c] > 0 && mem_in_use + cohort_keep[c] + mem_est <= budget)
+            return true;
+    return false;
+}
+
+// a queued job waits for memory (it would not fit the budget now)
+bool GtfServer::memory_blocked() const
+{
+    for (long id : pending)
+        if (mem_in_use + jobs[id].mem_est > budget) return true;
+    return false;
+}
+
+void GtfServer::start_job(long id, int cohort, double t)
+{
+    jobs[id].status = 1;
+    jobs[id].t_start = t;
+    jobs[id].cohort = cohort;
+}
+
+// a batch of the given jobs with the server's settings, its SCF begun
+static int open_batch(GtfServer* s, const std::vector<BasisSet_t>& bases, const std::vector<int>& sh, void* ctx,
+                      gtf_batch_t* out)
+{
+    gtf_batch_t b = nullptr;
+    int rc = gtf_batch_create(const_cast<BasisSet_t*>(bases.data()), (int)bases.size(), s->tolscr, &b);
+    // the mixed-precision form: FP32 ERIs below the Schwarz bound tau
+    if (rc == 0 && s->tau > 0.0) rc = gtf_batch_set_precision(b, s->tau);
+    if (rc == 0 && s->purif_mode != 0) rc = gtf_batch_set_purif_precision(b, s->purif_mode, s->purif_until);
+    if (rc == 0 && !sh.empty()) rc = gtf_batch_set_tile_shapes(b, sh.data());
+    if (rc == 0) rc = gtf_batch_scf_begin(b, s->max_iters, s->ediff, publish, ctx);
+    *out = b;
+    return rc;
 }
 
 void GtfServer::run_cohort(std::vector<long> ids, double mem, int cohort, bool large)
 {
     std::vector<BasisSet_t> bases;
     std::vector<int> sh;
+    int cls = 0;
     {
         std::lock_guard<std::mutex> lk(mu);
         const double t = swtime();
         int nbf_max = 0;
         for (long id : ids) {
             bases.push_back(jobs[id].basis);
-            jobs[id].status = 1;
-            jobs[id].t_start = t;
-            jobs[id].cohort = cohort;
+            start_job(id, cohort, t);
             nbf_max = std::max(nbf_max, jobs[id].nbf);
+            cls = jobs[id].size_class;
         }
+        // a cohort keeps its size class, so the shapes chosen here fit the
+        // molecules that join it later
         for (const ShapeRange& r : shape_ranges)
             if (nbf_max >= r.nbf_min && nbf_max <= r.nbf_max) sh = r.shapes;
     }
-    const int nmol = (int)ids.size();
-    std::vector<double> e(nmol, 0.0);
-    std::vector<int> it(nmol, 0);
-    CohortCtx ctx{this, &ids};
+    CohortCtx ctx{this, ids, 0.0};
     gtf_batch_t b = nullptr;
-    int rc = gtf_batch_create(bases.data(), nmol, tolscr, &b);
-    // the mixed-precision form: FP32 ERIs below the Schwarz bound tau
-    if (rc == 0 && tau > 0.0) rc = gtf_batch_set_precision(b, tau);
-    if (rc == 0 && !sh.empty()) rc = gtf_batch_set_tile_shapes(b, sh.data());
-    if (rc == 0) rc = gtf_batch_scf_cb(b, max_iters, ediff, e.data(), it.data(), nullptr, publish, &ctx);
+    int rc = open_batch(this, bases, sh, &ctx, &b);
+    int step = 0, last_regroup = -1000000;
+    double regroup_s = 0.0;
+    const double t_begin = swtime();
+    while (rc == 0) {
+        const int left = gtf_batch_scf_step(b);
+        if (left < 0) {
+            rc = left;
+            break;
+        }
+        step++;
+        // admission between iterations
+        std::vector<long> join;
+        std::vector<BasisSet_t> add;
+        double join_mem = 0.0, keep_mem = 0.0;
+        bool regroup = false, blocked_here = false;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (long id : ctx.ids)
+                if (jobs[id].status == 1) keep_mem += jobs[id].mem_est;
+            const double now = swtime();
+            const bool paced = step - last_regroup >= join_gap && regroup_s <= 0.5 * (now - t_begin - regroup_s);
+            if (join_gap >= 0 && (paced || left == 0) && !stopping) {
+                // while a regroup carries running molecules both batches
+                // exist, so those count twice against the budget; a cohort
+                // whose molecules have all ended frees its batch first
+                const double base = left > 0 ? mem_in_use + keep_mem : mem_in_use - ctx.ended_mem;
+                for (auto itp = pending.begin(); itp != pending.end() && left + (int)join.size() < max_jobs;) {
+                    const SJob& j = jobs[*itp];
+                    if (j.size_class != cls) {
+                        ++itp;
+                        continue;
+                    }
+                    if (base + join_mem + j.mem_est > budget) {
+                        blocked_here = true;
+                        ++itp;
+                        continue;
+                    }
+                    join.push_back(*itp);
+                    add.push_back(j.basis);
+                    join_mem += j.mem_est;
+                    itp = pending.erase(itp);
+                }
+            }
+            // without newcomers, a regroup that only drops finished molecules
+            // returns their memory to a job that waits for it (here or in the
+            // queue), if both batches fit the budget meanwhile
+            const bool shrink = left > 0 && ctx.ended_mem > 0 && mem_in_use + keep_mem <= budget &&
+                                (blocked_here || memory_blocked());
+            regroup = !join.empty() || shrink;
+            if (regroup) {
+                const double t = swtime();
+                for (long id : join) start_job(id, cohort, t);
+                mem_in_use += join_mem;
+                mem_peak = std::max(mem_peak, left > 0 ? mem_in_use + keep_mem : mem_in_use - ctx.ended_mem);
+                joins += (long)join.size();
+                regroups++;
+            }
+            cohort_room[cohort] = (left == 0 && join.empty()) ? -1 : max_jobs - left - (int)join.size();
+            cohort_keep[cohort] = keep_mem + join_mem;
+        }
+        if (!regroup) {
+            if (left == 0) break;
+            continue;
+        }
+        const double t0 = swtime();
+        gtf_batch_t nb = nullptr;
+        std::vector<long> nids;
+        if (left == 0) {
+            // nothing to carry over: free the finished batch, then open one
+            // for the newcomers
+            gtf_batch_destroy(b);
+            b = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                mem_in_use -= ctx.ended_mem;
+                mem -= ctx.ended_mem;
+                ctx.ended_mem = 0.0;
+                ctx.ids = join;
+                cv.notify_all();
+            }
+            rc = open_batch(this, add, sh, &ctx, &b);
+        } else {
+            std::vector<int> origin(left + join.size());
+            rc = gtf_batch_regroup(b, add.data(), (int)add.size(), &nb, origin.data());
+            std::lock_guard<std::mutex> lk(mu);
+            if (rc == 0) {
+                for (int o : origin) nids.push_back(o >= 0 ? ctx.ids[o] : join[-o - 1]);
+                gtf_batch_destroy(b);
+                b = nb;
+                ctx.ids = nids;
+                mem_in_use -= ctx.ended_mem;
+                mem -= ctx.ended_mem;
+                ctx.ended_mem = 0.0;
+                cv.notify_all();
+            } else {
+                // the newcomers fail with the cohort below
+                for (long id : join) ctx.ids.push_back(id);
+            }
+        }
+        mem += join_mem;
+        regroup_s += swtime() - t0;
+        last_regroup = step;
+    }
     gtf_batch_destroy(b);
     std::lock_guard<std::mutex> lk(mu);
     const double t = swtime();
-    for (int k = 0; k < nmol; k++) {
-        SJob& j = jobs[ids[k]];
+    for (long id : ctx.ids) {
+        SJob& j = jobs[id];
         if (j.status == 1) {
             // a failed cohort (or one that ended without reporting the job)
             j.status = rc < 0 ? rc : -1;
             j.t_done = t;
         }
     }
+    cohort_room[cohort] = -1;
     running--;
     if (large) running_large--;
     mem_in_use = running == 0 ? 0.0 : mem_in_use - mem;  // no rounding residue once idle
@@ -184,15 +351,20 @@ void GtfServer::schedule()
         if (pending.empty()) continue;
         // the oldest job opens the cohort (the oldest of a smaller class while
         // the large-molecule slots are all taken); same-class jobs join,
-        // oldest first
+        // oldest first.  Small-class jobs are left to a running cohort of
+        // their class that has room: it takes them at its next iteration.
         int cls = -1;
-        for (long id : pending)
-            if (jobs[id].size_class < LARGE_CLASS || running_large < max_large) {
-                cls = jobs[id].size_class;
+        for (long id : pending) {
+            const int c = jobs[id].size_class;
+            if (c < LARGE_CLASS && join_gap >= 0 && class_has_room(c, jobs[id].mem_est)) continue;
+            if (c < LARGE_CLASS || running_large < max_large) {
+                cls = c;
                 break;
             }
+        }
         if (cls < 0) {
-            // only large jobs are queued and their slots are taken
+            // only jobs that wait to join a cohort, or large jobs whose slots
+            // are taken, are queued
             cv.wait(lk);
             continue;
         }
@@ -226,6 +398,9 @@ void GtfServer::schedule()
         if (large) running_large++;
         peak_running = std::max(peak_running, running);
         const int cohort = ncohorts++;
+        cohort_class.push_back(cls);
+        cohort_room.push_back(max_jobs - (int)take.size());
+        cohort_keep.push_back(mem);
         // take out the threads of cohorts that have ended (each has already
         // published its results and returned its memory) and join them with
         // the lock released, so a long-running server holds no dead threads
@@ -278,6 +453,24 @@ int gtf_server_set_large_slots(gtf_server_t s, int n)
     if (!s || n < 1) return -3;
     std::lock_guard<std::mutex> lk(s->mu);
     s->max_large = std::min(n, s->max_cohorts);
+    s->cv.notify_all();
+    return 0;
+}
+
+int gtf_server_set_purif_precision(gtf_server_t s, int mode, double until)
+{
+    if (!s || mode < 0 || mode > 2 || until <= 0.0) return -3;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->purif_mode = mode;
+    s->purif_until = until;
+    return 0;
+}
+
+int gtf_server_set_join_gap(gtf_server_t s, int iters)
+{
+    if (!s) return -3;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->join_gap = iters;
     s->cv.notify_all();
     return 0;
 }
@@ -360,6 +553,15 @@ int gtf_server_stats(gtf_server_t s, double* out)
     out[6] = s->mem_peak;
     out[7] = s->budget;
     return 8;
+}
+
+int gtf_server_join_stats(gtf_server_t s, long* joins, long* regroups)
+{
+    if (!s) return -3;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (joins) *joins = s->joins;
+    if (regroups) *regroups = s->regroups;
+    return 0;
 }
 
 void gtf_server_destroy(gtf_server_t s)

@@ -106,13 +106,25 @@ int gtf_dgemm_host(int n, const double *A, const double *B, double *C, int trans
 /* The same; fp32 != 0 rounds A and B to FP32 and uses the FP32 MFMA. */
 int gtf_gemm_host(int n, const double *A, const double *B, double *C, int transB, int fp32);
 
-/* Mixed-precision purification (compute_purification): with fp32 != 0 the
- * D^2 and D^3 products use FP32 MFMA while ||D - D^2||_F >= fp32_until and
- * FP64 MFMA from then on, so the FP64 iterations correct the FP32 residual
- * and converge to the same projector; the final X D X stays FP64. */
+/* The split-BF16 product on host arrays: every row of A and of B^T is cut
+ * into ns (3..6) exact BF16 slices of 7 bits below its largest element, the
+ * slice products run on v_mfma_f32_32x32x16_bf16 with exact FP32 sums over
+ * 256-deep K chunks and are added in FP64; error about 2^(-7 ns) of the
+ * row maxima. */
+int gtf_gemm_split_host(int n, const double *A, const double *B, double *C, int transB, int ns);
+
+/* Mixed-precision purification (compute_purification): mode 1 uses FP32
+ * MFMA for the D^2 and D^3 products, mode 2 the split-BF16 product (FP64
+ * residuals carried slice to slice), while ||D - D^2||_F >= until, and FP64
+ * MFMA from then on, so the FP64 iterations restore idempotency at FP64
+ * resolution; mode 0 is FP64 throughout (the default).  The final X D X
+ * stays FP64.  Returns 0, -1 (invalid) or -2 (out of device memory). */
 struct _purif_t;
-int gtf_purif_set_precision(struct _purif_t *purif, int fp32, double fp32_until);
-/* purification iterations of the last compute_purification run in FP32 */
+int gtf_purif_set_precision(struct _purif_t *purif, int mode, double until);
+/* slices per operand row of the split-BF16 products (3..6, default 5) */
+int gtf_purif_set_slices(struct _purif_t *purif, int ns);
+/* purification iterations of the last compute_purification run in the
+ * reduced precision */
 int gtf_purif_fp32_iters(struct _purif_t *purif);
 
 /* ---- batches: many molecules per Fock build and per SCF iteration ------
@@ -136,8 +148,10 @@ long gtf_batch_total(gtf_batch_t batch);
 /* FP32 ERIs for quartets with Schwarz bound below tau (0: FP64 only) */
 int gtf_batch_set_precision(gtf_batch_t batch, double tau);
 /* mixed-precision purification inside gtf_batch_scf, per molecule, as
- * gtf_purif_set_precision */
-int gtf_batch_set_purif_precision(gtf_batch_t batch, int fp32, double fp32_until);
+ * gtf_purif_set_precision (mode 0 FP64, 1 FP32, 2 split BF16) */
+int gtf_batch_set_purif_precision(gtf_batch_t batch, int mode, double until);
+/* slices of the split-BF16 products (3..6, default 5) */
+int gtf_batch_set_purif_slices(gtf_batch_t batch, int ns);
 /* The tile autotuner on the batch's packed density, as gtf_pfock_autotune
  * (the next build re-captures its graph); shapes per class as
  * gtf_pfock_tile_shape / gtf_pfock_set_tile_shape, so the shapes tuned on
@@ -176,6 +190,29 @@ int gtf_batch_scf(gtf_batch_t batch, int max_iters, double ediff, double *energy
 typedef void (*gtf_scf_done_fn)(void *ctx, int m, double energy, int builds);
 int gtf_batch_scf_cb(gtf_batch_t batch, int max_iters, double ediff, double *energy, int *iters, double *history,
                      gtf_scf_done_fn done, void *ctx);
+/* The same SCF one iteration at a time, for serving loops that admit work
+ * between iterations.  gtf_batch_scf_begin resets every molecule's SCF and
+ * puts its initial density; each gtf_batch_scf_step runs one iteration of
+ * the molecules still running (molecules that have ended drop out of the
+ * Fock build) and calls done as each one ends; it returns how many are
+ * still running (0: all have ended), or < 0 on failure.
+ * gtf_batch_scf_results gives energy[m] (last build), iters[m] (builds of a
+ * molecule that has ended, 0 while it runs) and history[m * max_iters + k];
+ * any output may be NULL; returns nmol. */
+int gtf_batch_scf_begin(gtf_batch_t batch, int max_iters, double ediff, gtf_scf_done_fn done, void *ctx);
+int gtf_batch_scf_step(gtf_batch_t batch);
+int gtf_batch_scf_results(gtf_batch_t batch, double *energy, int *iters, double *history);
+/* Continuous batching: a new batch holding the molecules of `batch` whose
+ * SCF is still running -- with their SCF state carried over exactly
+ * (density, DIIS vectors and B matrix, iteration counts, energies) -- and
+ * nadd new molecules that start their SCF in it (initial density, their own
+ * iteration count 0), with batch's screening tolerance, precision, tile
+ * shapes and callback.  origin[k] (may be NULL; capacity nrun + nadd) tells
+ * where molecule k of the new batch came from: its index in `batch`, or
+ * -(j + 1) for add[j].  The caller destroys `batch` afterwards, which
+ * returns the memory of its finished molecules.  Returns 0, -1, -2 or -3
+ * (no SCF running in batch, or nothing to carry or add). */
+int gtf_batch_regroup(gtf_batch_t batch, BasisSet_t *add, int nadd, gtf_batch_t *out, int *origin);
 /* wall-clock time (gettimeofday seconds) at which each molecule's last
  * gtf_batch_scf ended; returns nmol */
 int gtf_batch_done_times(gtf_batch_t batch, double *t);
@@ -189,12 +226,16 @@ double gtf_batch_job_bytes(BasisSet_t basis);
  * of its size class (basis functions within the same power-of-two band),
  * oldest first, up to max_jobs_per_cohort and the device-memory budget
  * (gtf_batch_job_bytes per job).  Each cohort runs as one batch
- * (gtf_batch_scf_cb) on its own thread and streams, up to max_cohorts at
- * once, of which at most half hold molecules of 384 or more basis functions
- * (the oldest job of a smaller class opens the next cohort while those slots
- * are taken); a job's result is published when its own SCF ends, its memory
- * returns to the budget when its cohort ends.  The BasisSet of a job must
- * stay alive until the job is done. */
+ * (gtf_batch_scf_step, one SCF iteration at a time) on its own thread and
+ * streams, up to max_cohorts at once, of which at most half hold molecules
+ * of 384 or more basis functions (the oldest job of a smaller class opens
+ * the next cohort while those slots are taken).  Between iterations a
+ * cohort admits queued jobs of its size class that fit the budget
+ * (gtf_batch_regroup: running molecules carry their SCF state over, new
+ * ones start, finished ones leave and return their memory); while a
+ * small-class cohort has room, its class's queued jobs wait for it instead
+ * of opening another cohort.  A job's result is published when its own SCF
+ * ends.  The BasisSet of a job must stay alive until the job is done. */
 typedef struct GtfServer *gtf_server_t;
 
 int gtf_server_create(double hbm_budget_bytes, int max_cohorts, int max_jobs_per_cohort, double tolscr, double tau,
@@ -212,6 +253,12 @@ int gtf_server_set_tile_shapes(gtf_server_t server, const int *shapes);
  * last matching range wins; gtf_server_set_tile_shapes clears the ranges).
  * Returns 0 or -3. */
 int gtf_server_set_tile_shapes_range(gtf_server_t server, int nbf_min, int nbf_max, const int *shapes);
+/* purification products of the cohorts started after the call, as
+ * gtf_batch_set_purif_precision (mode 0 FP64, 1 FP32, 2 split BF16) */
+int gtf_server_set_purif_precision(gtf_server_t server, int mode, double until);
+/* SCF iterations between two admissions into a running cohort (default 1;
+ * a negative value: cohorts never admit jobs while they run) */
+int gtf_server_set_join_gap(gtf_server_t server, int iters);
 /* returns the job id (>= 0) or -3 */
 long gtf_server_submit(gtf_server_t server, BasisSet_t basis);
 /* status: 0 queued, 1 running, 2 done, < 0 failed (-2: larger than the
@@ -223,6 +270,8 @@ int gtf_server_wait(gtf_server_t server);
 /* out[8] = {jobs, queued, running cohorts, cohorts started, peak running
  *           cohorts, bytes in use, peak bytes, budget} */
 int gtf_server_stats(gtf_server_t server, double *out);
+/* jobs admitted into a running cohort, and regroups (each may be NULL) */
+int gtf_server_join_stats(gtf_server_t server, long *joins, long *regroups);
 /* cancels queued jobs, waits for running cohorts */
 void gtf_server_destroy(gtf_server_t server);
 

@@ -18,8 +18,16 @@ struct BasisSet {
     double energy;
 };
 
+// a molecule's SCF runs in iterations of STEP_S seconds and ends once it
+// has run its `seconds`
+static const double STEP_S = 0.01;
+
 struct GtfBatch {
     std::vector<BasisSet*> mols;
+    std::vector<double> elapsed;  // seconds of SCF run so far, per molecule
+    std::vector<int> running;
+    gtf_scf_done_fn done = nullptr;
+    void* ctx = nullptr;
 };
 
 static std::atomic<int> g_alive{0}, g_max_alive{0}, g_created{0};
@@ -68,20 +76,70 @@ int gtf_batch_set_tile_shapes(gtf_batch_t, const int* shapes)
 }
 int gtf_batch_set_purif_precision(gtf_batch_t, int, double) { return 0; }
 
-int gtf_batch_scf_cb(gtf_batch_t b, int, double, double* energy, int* iters, double*, gtf_scf_done_fn done,
-                     void* ctx)
+int gtf_batch_scf_begin(gtf_batch_t b, int, double, gtf_scf_done_fn done, void* ctx)
 {
-    const int n = (int)b->mols.size();
-    std::vector<int> order(n);
-    for (int m = 0; m < n; m++) order[m] = m;
-    std::sort(order.begin(), order.end(), [&](int x, int y) { return b->mols[x]->seconds < b->mols[y]->seconds; });
-    const auto t0 = std::chrono::steady_clock::now();
-    for (int m : order) {
-        std::this_thread::sleep_until(t0 + std::chrono::duration<double>(b->mols[m]->seconds));
-        energy[m] = b->mols[m]->energy;
-        iters[m] = 10;
-        if (done) done(ctx, m, energy[m], 10);
+    b->done = done;
+    b->ctx = ctx;
+    b->elapsed.assign(b->mols.size(), 0.0);
+    b->running.assign(b->mols.size(), 1);
+    return 0;
+}
+
+static std::atomic<int> g_steps{0}, g_regroups{0};
+
+int gtf_batch_scf_step(gtf_batch_t b)
+{
+    std::this_thread::sleep_for(std::chrono::duration<double>(STEP_S));
+    g_steps++;
+    int left = 0;
+    for (size_t m = 0; m < b->mols.size(); m++) {
+        if (!b->running[m]) continue;
+        b->elapsed[m] += STEP_S;
+        if (b->elapsed[m] >= b->mols[m]->seconds - 1e-9) {
+            b->running[m] = 0;
+            if (b->done) b->done(b->ctx, (int)m, b->mols[m]->energy, (int)(b->elapsed[m] / STEP_S + 0.5));
+        } else {
+            left++;
+        }
     }
+    return left;
+}
+
+int gtf_batch_scf_results(gtf_batch_t b, double* energy, int* iters, double*)
+{
+    for (size_t m = 0; m < b->mols.size(); m++) {
+        if (energy) energy[m] = b->mols[m]->energy;
+        if (iters) iters[m] = b->running[m] ? 0 : 10;
+    }
+    return (int)b->mols.size();
+}
+
+int gtf_batch_regroup(gtf_batch_t b, BasisSet_t* add, int nadd, gtf_batch_t* out, int* origin)
+{
+    GtfBatch* nb = new GtfBatch();
+    nb->done = b->done;
+    nb->ctx = b->ctx;
+    int k = 0;
+    for (size_t m = 0; m < b->mols.size(); m++)
+        if (b->running[m]) {
+            nb->mols.push_back(b->mols[m]);
+            nb->elapsed.push_back(b->elapsed[m]);
+            nb->running.push_back(1);
+            if (origin) origin[k] = (int)m;
+            k++;
+        }
+    for (int j = 0; j < nadd; j++) {
+        nb->mols.push_back(add[j]);
+        nb->elapsed.push_back(0.0);
+        nb->running.push_back(1);
+        if (origin) origin[k] = -(j + 1);
+        k++;
+    }
+    const int a = ++g_alive;
+    int prev = g_max_alive.load();
+    while (a > prev && !g_max_alive.compare_exchange_weak(prev, a)) {}
+    g_regroups++;
+    *out = nb;
     return 0;
 }
 
@@ -90,6 +148,8 @@ BasisSet* stub_molecule(int nbf, double seconds, double energy) { return new Bas
 void stub_free(BasisSet* b) { delete b; }
 int stub_max_alive() { return g_max_alive.load(); }
 int stub_created() { return g_created.load(); }
+int stub_regroups() { return g_regroups.load(); }
+int stub_steps() { return g_steps.load(); }
 int stub_shapes_set() { return g_shapes_set.load(); }
 int stub_shape0_units() { return g_shape0_units.load(); }
 int stub_shape_units(int* out, int cap)
@@ -103,6 +163,8 @@ void stub_reset()
 {
     g_max_alive = 0;
     g_created = 0;
+    g_regroups = 0;
+    g_steps = 0;
     g_shapes_set = 0;
     g_shape0_units = 0;
     {

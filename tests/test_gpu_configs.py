@@ -1,0 +1,228 @@
+"""The workloads of the measurement at their real sizes, through the C-ABI.
+
+* C10H22 / cc-pVDZ (250 basis functions): the GPU's 15-iteration FP64 SCF
+  against the oracle's energies (tests/golden/scf_c10h22_oracle.json,
+  written by tests/golden/make_oracle_scf.py), every iteration within
+  1e-9 Eh.
+* C54H18 / cc-pVDZ (846 basis functions): FP64 ERIs of sampled shell
+  quartets against the oracle (1e-12); the mixed form -- FP32 ERIs below a
+  Schwarz bound of 1e-4 and split-BF16 purification -- against the plain
+  FP64 SCF: the converged energies within 1e-6 Eh (the per-iteration
+  differences of the converged tail likewise), and the final density
+  idempotent with tr(DS) = nocc at n = 846.  (The oracle itself would take
+  about ten minutes per C54H18 Fock build on the host, so the FP64 GPU path,
+  pinned to the oracle on the molecules above, is the reference here.)
+* the seeded 256-molecule batch (all-trans alkanes C1..C20, water clusters
+  of 1..16 monomers, seed 20261202): a sample of molecules' G from the
+  batched build against single-molecule builds (1e-10), and the batched SCF
+  from the atomic-block guess in the mixed form (FP32 ERIs below 1e-4,
+  split-BF16 purification): every molecule converges in both forms, within
+  1e-6 Eh of the FP64 batched SCF.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+gpu = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def g():
+    import gtfock_amd
+    gtfock_amd.lib()
+    return gtfock_amd
+
+
+@gpu
+def test_c10h22_scf_matches_oracle_energies(g, gbs, mol_dir):
+    gold = json.load(open(os.path.join(HERE, "golden", "scf_c10h22_oracle.json")))
+    eo = np.array(gold["energies"])
+    e, _, _ = g.scf(gbs, str(mol_dir / "c10h22.xyz"), gold["iterations"], tolscr=gold["tolscr"])
+    n = min(len(e), len(eo))
+    d = np.abs(e[:n] - eo[:n])
+    print("C10H22 SCF: GPU %.12f, oracle %.12f over %d iterations; max |dE| %.2e" % (e[n - 1], eo[n - 1], n, d.max()))
+    assert n == len(eo)
+    assert d.max() < 1e-9
+
+
+@gpu
+def test_c54h18_sampled_eris_match_oracle(g, gbs, mol_dir):
+    xyz = str(mol_dir / "c54h18.xyz")
+    b = g.BasisSet(gbs, xyz)
+    ob = oracle.Basis(gbs, xyz)
+    assert b.nbf == 846
+    rng = np.random.default_rng(54)
+    ns = b.nshells
+    # 300 random quartets plus near ones (shells of neighbouring atoms)
+    q = rng.integers(0, ns, size=(300, 4))
+    near = rng.integers(0, ns - 12, size=(100, 1)) + rng.integers(0, 12, size=(100, 4))
+    q = np.concatenate([q, near]).astype(np.int32)
+    erd = g.ERD(b)
+    got = erd.quartets(q)
+    worst = 0.0
+    for k, (M, N, P, Q) in enumerate(q):
+        ref = ob.eri_quartet(int(M), int(N), int(P), int(Q)).reshape(-1)
+        worst = max(worst, float(np.abs(got[k] - ref).max()))
+    print("C54H18: %d sampled quartets, max |ERI_gpu - ERI_oracle| %.2e" % (len(q), worst))
+    assert worst < 1e-12
+
+
+@gpu
+def test_c54h18_mixed_scf_against_fp64(g, gbs, mol_dir):
+    xyz = str(mol_dir / "c54h18.xyz")
+    e64, D64, _ = g.scf(gbs, xyz, 40, ediff=1e-8)
+    st = {}
+    em, Dm, _ = g.scf(gbs, xyz, 40, ediff=1e-8, tau=1e-4, purif="split", stats=st)
+    print("C54H18 FP64 SCF: %d builds, E = %.10f" % (len(e64), e64[-1]))
+    print("C54H18 mixed SCF (FP32 ERIs below 1e-4, split-BF16 purification): %d builds, E = %.10f" % (
+        len(em), em[-1]))
+    n = min(len(e64), len(em))
+    for k in range(n):
+        print("  build %2d: FP64 %.10f mixed %.10f dE %+.2e" % (k, e64[k], em[k], em[k] - e64[k]))
+    print("  split-BF16 purification iterations per SCF iteration: %s" % st["purif_low_iters"])
+    assert len(e64) < 40 and len(em) < 40
+    assert abs(em[-1] - e64[-1]) < 1e-6
+    # the converged tail: builds whose energy changed by less than 1e-6 Eh
+    tail = [k for k in range(1, n) if abs(e64[k] - e64[k - 1]) < 1e-6]
+    assert len(tail) >= 2
+    assert max(abs(em[k] - e64[k]) for k in tail) < 1e-6
+    assert min(st["purif_low_iters"]) > 0
+    b = g.BasisSet(gbs, xyz)
+    pf = g.PFock(b)
+    S = pf.overlap()
+    pf.close()
+    DS = Dm @ S
+    assert np.abs(DS @ DS - DS).max() < 1e-8
+    assert abs(np.trace(DS) - b.nocc) < 1e-8
+
+
+@pytest.fixture(scope="module")
+def batch256(g, gbs, tmp_path_factory):
+    from gtfock_amd import geometry
+    d = tmp_path_factory.mktemp("batch256")
+    paths = []
+    for k, (name, atoms) in enumerate(geometry.batch_molecules(256, 20261202)):
+        p = str(d / ("m%03d.xyz" % k))
+        open(p, "w").write(geometry.to_xyz(atoms))
+        paths.append(p)
+    return paths
+
+
+@gpu
+def test_batch256_fock_matches_single_molecules(g, gbs, batch256):
+    bases = [g.BasisSet(gbs, p) for p in batch256]
+    nbf = [b.nbf for b in bases]
+    assert min(nbf) == 24 and max(nbf) >= 450
+    bt = g.Batch(bases)
+    rng = np.random.default_rng(256)
+    Ds = []
+    for b in bases:
+        A = rng.normal(size=(b.nbf, b.nbf)) * 0.02
+        Ds.append(A + A.T)
+    Gs = bt.fock(Ds)
+    sample = sorted(set([int(np.argmin(nbf)), int(np.argmax(nbf))] + list(rng.integers(0, 256, size=6))))
+    for m in sample:
+        pf = g.PFock(bases[m], tolscr=1e-11)
+        G1 = pf.fock(Ds[m])
+        pf.close()
+        err = np.abs(Gs[m] - G1).max()
+        print("molecule %d (nbf %d): max |G_batch - G_single| %.2e" % (m, nbf[m], err))
+        assert err < 1e-10
+    bt.close()
+
+
+@pytest.fixture(scope="module")
+def guess_gbs(g, gbs, tmp_path_factory):
+    """cc-pVDZ beside generated <Element>.dat atomic guesses, so that the
+    batched SCF starts from scf.c's atomic-block density."""
+    import shutil
+    from gtfock_amd import guess
+    d = tmp_path_factory.mktemp("guess")
+    p = str(d / "cc-pvdz.gbs")
+    shutil.copy(gbs, p)
+    guess.write_dat(gbs, str(d), ("H", "C", "O"))
+    return p
+
+
+@gpu
+def test_batch256_mixed_scf_against_fp64(g, guess_gbs, batch256):
+    bases = [g.BasisSet(guess_gbs, p) for p in batch256]
+    b64 = g.Batch(bases)
+    e64, it64 = b64.scf(max_iters=30, ediff=1e-8)
+    b64.close()
+    bm = g.Batch(bases, tau=1e-4)
+    bm.set_purif_precision("split")
+    em, itm = bm.scf(max_iters=30, ediff=1e-8)
+    bm.close()
+    both = [m for m in range(len(bases)) if 0 < it64[m] < 30 and 0 < itm[m] < 30]
+    d = np.abs(em[both] - e64[both])
+    print("256-molecule batch: %d converged in both forms (FP64 %d, mixed %d); max |dE| %.2e Eh, mean builds "
+          "FP64 %.1f mixed %.1f" % (len(both), int(((it64 > 0) & (it64 < 30)).sum()),
+                                    int(((itm > 0) & (itm < 30)).sum()), d.max(), it64[both].mean(),
+                                    itm[both].mean()))
+    assert len(both) == len(bases)
+    assert d.max() < 1e-6
+
+
+@gpu
+def test_serving_stream_with_flakes_to_c96h24(g, guess_gbs, tmp_path):
+    """The serving workload: 16 requests of the seeded stream (seed 20261203;
+    70 % alkanes / water clusters, 30 % flakes C24H12..C96H24) arriving as a
+    Poisson process of 50 jobs/s, in the mixed form (FP32 ERIs below 1e-4,
+    split-BF16 purification), continuous batching on.  Every job completes
+    (converged at |dE| < 1e-8 Eh), within 1e-6 Eh of the FP64 SCF of the same
+    molecule; a flake and a small molecule give the same energy alone as in
+    the mix (1e-8 Eh)."""
+    import time
+    from gtfock_amd import geometry
+    mols = geometry.serve_molecules(16, seed=20261203, max_rings=4)
+    names = [n for n, _ in mols]
+    assert "C96H24" in names
+    paths = []
+    for k, (name, atoms) in enumerate(mols):
+        p = str(tmp_path / ("job%02d.xyz" % k))
+        open(p, "w").write(geometry.to_xyz(atoms))
+        paths.append(p)
+    bases = [g.BasisSet(guess_gbs, p) for p in paths]
+    arrivals = np.cumsum(np.random.default_rng(20261203).exponential(1.0 / 50.0, size=len(bases)))
+    s = g.Server(hbm_budget_gb=200.0, max_cohorts=16, max_jobs_per_cohort=64, tau=1e-4, max_iters=40, ediff=1e-8,
+                 purif="split")
+    s.set_large_slots(2)
+    t0 = time.time()
+    ids = []
+    for b, t in zip(bases, arrivals):
+        time.sleep(max(0.0, t0 + t - time.time()))
+        ids.append(s.submit(b))
+    s.wait()
+    mix = [s.poll(i) for i in ids]
+    joins, regroups = s.join_stats()
+    s.close()
+    lat = np.array([r["t_done"] - r["t_submit"] for r in mix])
+    print("served 16 jobs: p50 %.2f s, p99 %.2f s, makespan %.1f s, joins %d, regroups %d" % (
+        np.percentile(lat, 50), np.percentile(lat, 99), max(r["t_done"] for r in mix) - t0, joins, regroups))
+    assert all(r["status"] == 2 for r in mix), [(n, r["status"], r["iters"]) for n, r in zip(names, mix)]
+    # FP64 reference energies of the same molecules, one FP64 batch
+    ref = g.Batch(bases)
+    e64, it64 = ref.scf(max_iters=40, ediff=1e-8)
+    ref.close()
+    for n, r, e, it in zip(names, mix, e64, it64):
+        print("  %-8s mixed %.10f (%d builds)  FP64 %.10f (%d builds)  dE %.1e" % (
+            n, r["energy"], r["iters"], e, it, r["energy"] - e))
+    assert (it64 > 0).all() and (it64 < 40).all()
+    assert np.abs(np.array([r["energy"] for r in mix]) - e64).max() < 1e-6
+    # alone = in the mix, for the first C54H18 and the first small job
+    for k in (names.index("C54H18"), 0):
+        s1 = g.Server(hbm_budget_gb=200.0, max_cohorts=1, max_jobs_per_cohort=1, tau=1e-4, max_iters=40,
+                      ediff=1e-8, purif="split", join_gap=-1)
+        jid = s1.submit(bases[k])
+        s1.wait()
+        a = s1.poll(jid)
+        s1.close()
+        print("  %s alone %.12f (%d builds), in the mix %.12f (%d builds)" % (
+            names[k], a["energy"], a["iters"], mix[k]["energy"], mix[k]["iters"]))
+        assert abs(a["energy"] - mix[k]["energy"]) < 1e-8

@@ -245,3 +245,78 @@ def test_batched_scf_starts_from_the_atomic_guess(g, gbs, mol_dir, tmp_path):
             os.path.basename(x), e[k], it[k], es[-1], len(es), ez[k], itz[k]))
         assert abs(e[k] - es[-1]) < 1e-9
         assert abs(e[k] - ez[k]) < 1e-8
+
+
+@gpu
+def test_batch_regroup_carries_scf_state(g, gbs, small_batch):
+    """A batch regrouped after three iterations -- its running molecules'
+    SCF state carried into a new batch, a new molecule starting there --
+    gives every molecule the energies and build counts of an SCF that ran
+    without the regroup."""
+    bases = [g.BasisSet(gbs, p) for p in small_batch]
+    first, newcomer = bases[1:4], bases[4]
+    ref = g.Batch(first + [newcomer])
+    e_ref, it_ref = ref.scf(max_iters=25, ediff=1e-10)
+    ref.close()
+    bt = g.Batch(first)
+    bt.scf_begin(max_iters=25, ediff=1e-10)
+    for _ in range(3):
+        assert bt.scf_step() == len(first)
+    nb, origin = bt.regroup([newcomer])
+    bt.close()
+    assert list(origin) == [0, 1, 2, -1]
+    while nb.scf_step() > 0:
+        pass
+    e, it, hist = nb.scf_results()
+    nb.close()
+    for k in range(4):
+        print("molecule %d: regrouped %.12f (%d builds), plain %.12f (%d builds)" % (k, e[k], it[k], e_ref[k], it_ref[k]))
+    assert np.abs(e - e_ref).max() < 1e-9
+    assert list(it) == list(it_ref)
+    # the carried molecules kept their first three energies
+    assert not np.isnan(hist[:3, :3]).any()
+
+
+@gpu
+def test_server_job_joins_running_cohort(g, gbs, tmp_path):
+    """Jobs submitted while a cohort of their size class runs join it between
+    iterations (same cohort, no new one) and get the energies and build
+    counts they get alone."""
+    import time
+    from gtfock_amd import geometry as geo
+    mols = [("c10h22", geo.alkane(10)), ("c8h18", geo.alkane(8)), ("w9", geo.water_cluster(9, np.random.default_rng(9)))]
+    paths = []
+    for name, atoms in mols:
+        p = str(tmp_path / (name + ".xyz"))
+        open(p, "w").write(geo.to_xyz(atoms))
+        paths.append(p)
+    bases = [g.BasisSet(gbs, p) for p in paths]
+    assert all(192 <= b.nbf < 384 for b in bases)  # one size class
+    alone = []
+    for b in bases:
+        s = g.Server(hbm_budget_gb=32.0, max_cohorts=1, max_jobs_per_cohort=1, max_iters=30, ediff=1e-9,
+                     join_gap=-1)
+        jid = s.submit(b)
+        s.wait()
+        alone.append(s.poll(jid))
+        s.close()
+    s = g.Server(hbm_budget_gb=32.0, max_cohorts=4, max_jobs_per_cohort=8, max_iters=30, ediff=1e-9)
+    ids = [s.submit(bases[0])]
+    while s.poll(ids[0])["status"] == 0:
+        time.sleep(0.001)
+    ids += [s.submit(b) for b in bases[1:]]
+    s.wait()
+    mix = [s.poll(i) for i in ids]
+    joins, regroups = s.join_stats()
+    st = s.stats()
+    s.close()
+    for k, (a, r) in enumerate(zip(alone, mix)):
+        print("job %d: alone %.12f (%d builds), joined %.12f (%d builds, cohort %d, started %.3f s after the first)" % (
+            k, a["energy"], a["iters"], r["energy"], r["iters"], r["cohort"], r["t_start"] - mix[0]["t_start"]))
+    print("joins %d, regroups %d, cohorts %d" % (joins, regroups, st["cohorts"]))
+    assert all(r["status"] == 2 for r in mix)
+    assert joins == 2 and st["cohorts"] == 1
+    assert all(r["t_start"] < mix[0]["t_done"] for r in mix[1:])
+    for a, r in zip(alone, mix):
+        assert abs(a["energy"] - r["energy"]) < 1e-9
+        assert a["iters"] == r["iters"]

@@ -61,6 +61,31 @@ def test_pack_unpack_roundtrip(g, gbs, mol_dir):
     ctypes.CDLL(None).free(buf)
 
 
+def test_imported_basis_guess_is_read_only_zero(g):
+    """CInt_importBasisSet: an imported basis has no guess blocks, so
+    CInt_getInitialGuess hands out a zero block without writing the basis;
+    element numbers outside the table are rejected."""
+    L = g.lib()
+    h = ctypes.c_void_p()
+    assert L.CInt_createBasisSet(ctypes.byref(h)) == 0
+    I = ctypes.c_int * 2
+    D = ctypes.c_double * 2
+    Zs, X, Y, Z = I(1, 1), D(0.0, 0.0), D(0.0, 0.0), D(0.0, 1.4)
+    spa, pps, Ls = I(1, 1), I(1, 1), I(0, 0)
+    cc, al = D(1.0, 1.0), D(0.5, 0.5)
+    args = [h, 2, Zs, X, Y, Z, 2, 2, 1, spa, pps, Ls, cc, al]
+    assert L.CInt_importBasisSet(*args) == 0
+    guess = ctypes.POINTER(ctypes.c_double)()
+    s, e = ctypes.c_int(), ctypes.c_int()
+    for a in range(2):
+        L.CInt_getInitialGuess(h, a, ctypes.byref(guess), ctypes.byref(s), ctypes.byref(e))
+        assert (s.value, e.value) == (a, a) and guess[0] == 0.0
+    bad = list(args)
+    bad[2] = I(1, 99)
+    assert L.CInt_importBasisSet(*bad) == 3  # CINT_STATUS_INVALID_VALUE
+    L.CInt_destroyBasisSet(h)
+
+
 def test_charge_line_and_guess(g, gbs, tmp_path):
     xyz = tmp_path / "oh.xyz"
     xyz.write_text("2\n-1\nO 0 0 0\nH 0 0 0.97\n")

@@ -1,7 +1,9 @@
 """Host tests of the request server's scheduling (gtfock_amd/csrc/serve.cpp)
-built against a stand-in batch backend (tests/sched_stub.cpp): size-class
-cohorts, per-job publication, the memory budget, the cohort limit, jobs
-that can never fit, and cancellation.  No GPU is used."""
+built against a stand-in batch backend (tests/sched_stub.cpp, where a
+molecule's SCF runs in 10 ms iterations): size-class cohorts, per-job
+publication, the memory budget, the cohort limit, jobs that can never fit,
+cancellation, and continuous batching (jobs joining running cohorts between
+iterations, finished jobs returning memory early).  No GPU is used."""
 import ctypes
 import os
 import subprocess
@@ -32,6 +34,8 @@ def lib(tmp_path_factory):
     L.stub_molecule.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.stub_molecule.restype = vp
     L.stub_cohort_sizes.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    L.gtf_server_set_join_gap.argtypes = [vp, ctypes.c_int]
+    L.gtf_server_join_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
     return L
 
 
@@ -157,6 +161,7 @@ def test_ended_cohort_threads_are_reclaimed(lib):
     L = lib
     L.stub_reset()
     h = server(L, 1e12, 1, 1)
+    assert L.gtf_server_set_join_gap(h, -1) == 0  # a cohort never takes a second job
     mols = [L.stub_molecule(30, 0.001, -1.0) for _ in range(4)]
     ids = [L.gtf_server_submit(h, mols[k % 4]) for k in range(8)]
     L.gtf_server_wait(h)
@@ -242,4 +247,76 @@ def test_large_slots_setting(lib):
     assert spans[1][0] >= spans[0][1] - 0.01 and spans[2][0] >= spans[1][1] - 0.01
     done_small = max(poll(L, h, i)[1][4] for i in ids_small)
     assert done_small < spans[0][1]  # the small jobs did not wait for them
+    L.gtf_server_destroy(h)
+
+
+def join_stats(L, h):
+    j, r = ctypes.c_long(), ctypes.c_long()
+    assert L.gtf_server_join_stats(h, ctypes.byref(j), ctypes.byref(r)) == 0
+    return j.value, r.value
+
+
+def test_jobs_join_a_running_cohort(lib):
+    """Jobs that arrive while a cohort of their size class runs join it at
+    its next iteration instead of waiting for it or opening a new cohort."""
+    L = lib
+    L.stub_reset()
+    h = server(L, 1e12, 4, 64)
+    first = L.stub_molecule(30, 0.4, -1.0)
+    later = [L.stub_molecule(30, 0.1, -2.0 - k) for k in range(3)]
+    ids = [L.gtf_server_submit(h, first)]
+    time.sleep(0.1)
+    ids += [L.gtf_server_submit(h, m) for m in later]
+    L.gtf_server_wait(h)
+    res = [poll(L, h, i) for i in ids]
+    assert all(st == 2 for st, _ in res)
+    assert [r[0] for _, r in res] == [-1.0, -2.0, -3.0, -4.0]
+    cohorts = {int(r[5]) for _, r in res}
+    assert cohorts == {0} and stats(L, h)["cohorts"] == 1
+    joins, regroups = join_stats(L, h)
+    assert joins == 3 and 1 <= regroups <= 3
+    # the newcomers ran beside the first job: they started and ended before it ended
+    t_done_first = res[0][1][4]
+    assert all(r[3] < t_done_first and r[4] < t_done_first for _, r in res[1:])
+    L.gtf_server_destroy(h)
+
+
+def test_finished_jobs_return_memory_between_iterations(lib):
+    """Finished jobs' memory returns to the budget between iterations, not
+    when their cohort ends: a queued job that needs it starts while the long
+    job of that cohort still runs, and the budget holds throughout (both
+    batches of a regroup counted)."""
+    L = lib
+    L.stub_reset()
+    one = 1000.0 * 30 * 30
+    h = server(L, 5.0 * one, 4, 64)
+    first = [L.stub_molecule(30, 0.05, -1.0 - k) for k in range(3)] + [L.stub_molecule(30, 0.6, -9.0)]
+    ids = [L.gtf_server_submit(h, m) for m in first]
+    time.sleep(0.02)
+    late = L.stub_molecule(42, 0.05, -5.0)  # about two jobs' memory: fits only after the short jobs leave
+    ids.append(L.gtf_server_submit(h, late))
+    L.gtf_server_wait(h)
+    res = [poll(L, h, i) for i in ids]
+    st = stats(L, h)
+    assert all(s_ == 2 for s_, _ in res)
+    assert st["peak_bytes"] <= st["budget"]
+    t_done_long = res[3][1][4]
+    assert res[4][1][3] < t_done_long  # the late job started while the long job ran
+    assert join_stats(L, h)[1] >= 1
+    L.gtf_server_destroy(h)
+
+
+def test_join_gap_off_keeps_cohorts_closed(lib):
+    L = lib
+    L.stub_reset()
+    h = server(L, 1e12, 4, 64)
+    assert L.gtf_server_set_join_gap(h, -1) == 0
+    first = L.stub_molecule(30, 0.2, -1.0)
+    later = L.stub_molecule(30, 0.05, -2.0)
+    ids = [L.gtf_server_submit(h, first)]
+    time.sleep(0.05)
+    ids.append(L.gtf_server_submit(h, later))
+    L.gtf_server_wait(h)
+    assert join_stats(L, h)[0] == 0
+    assert int(poll(L, h, ids[0])[1][5]) != int(poll(L, h, ids[1])[1][5])
     L.gtf_server_destroy(h)
