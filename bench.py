@@ -73,17 +73,37 @@ def first_density(g, pf, basis):
     return D
 
 
-def cpu_baseline(gbs, xyz, D, tol):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(gbs, xyz, D, tol, builds=3):
     """The oracle's FP64 Fock build (OpenMP, the reference's task structure)
-    on this host's cores, one full build of the same molecule and density."""
+    on this host's cores: `builds` builds of the same molecule and density;
+    the value is the median of the builds' quartet loops + reduction +
+    symmetrisation, the screening (CSR) pass of each build timed apart and
+    left out, as GTFock screens once per PFock_create."""
     import oracle
     b = oracle.Basis(gbs, xyz)
-    t0 = time.perf_counter()
-    b.fock(D, tol=tol)
-    dt = time.perf_counter() - t0
+    build_s, screen_s = [], []
+    for _ in range(builds):
+        _, _, st = b.fock(D, tol=tol)
+        build_s.append(st["build_s"])
+        screen_s.append(st["screen_s"])
+    dt = float(np.median(build_s))
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": 1.0 / dt, "unit": "Fock builds/s", "cores": threads, "kind": "port",
-            "sample": "one full FP64 Fock build of the same molecule and density (%.1f s)" % dt}
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "sample": "%d full FP64 Fock builds of the same molecule and density with %d OpenMP threads; median "
+                      "build %.2f s (builds %s s), screening pass %.2f s per build timed apart, not counted" % (
+                          builds, threads, dt, ", ".join("%.2f" % x for x in build_s), float(np.median(screen_s)))}
 
 
 def batch_bases(g, gbs, nmol, seed):
@@ -302,7 +322,8 @@ def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters, tune_nmol, tune_reps, t
     return out
 
 
-def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max_iters, ref, shapes=(), large=0):
+def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max_iters, ref, shapes=(), large=0,
+              purif="split", join_gap=1):
     """SCF requests with seeded Poisson arrivals (rate jobs/s) submitted in
     real time to the request server (mixed precision, HBM-budget admission,
     several cohorts on the device at once).  Time to converge = done time -
@@ -322,7 +343,7 @@ def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max
     rng = np.random.default_rng(seed + 1)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=njobs))
     srv = g.Server(hbm_budget_gb=budget_gb, max_cohorts=cohorts, max_jobs_per_cohort=64, tau=tau,
-                   max_iters=max_iters, ediff=1e-8)
+                   max_iters=max_iters, ediff=1e-8, purif=purif, join_gap=join_gap)
     for lo, hi, sh in shapes:
         srv.set_tile_shapes_range(lo, hi, sh)
     if large > 0:
@@ -349,6 +370,7 @@ def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max
     srv.wait()
     res = [srv.poll(i) for i in ids]
     st = srv.stats()
+    joins, regroups = srv.join_stats()
     srv.close()
     ok = [r for r in res if r["status"] == 2]
     lat = np.array([r["t_done"] - r["t_submit"] for r in ok])
@@ -356,7 +378,8 @@ def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max
     nbf = [b.nbf for b in bases]
     out = {"jobs": njobs, "completed": len(ok), "rate_jobs_per_s": rate, "seed": seed,
            "flake_rings_max": max_rings, "nbf_max": max(nbf), "nbf_mean": float(np.mean(nbf)),
-           "tau": tau, "hbm_budget_gb": budget_gb, "max_cohorts": cohorts, "cohorts": int(st["cohorts"]),
+           "tau": tau, "purification": purif, "join_gap": join_gap, "joins": joins, "regroups": regroups,
+           "hbm_budget_gb": budget_gb, "max_cohorts": cohorts, "cohorts": int(st["cohorts"]),
            "peak_running_cohorts": int(st["peak_running"]), "peak_gb": st["peak_bytes"] / 2 ** 30,
            "throughput_jobs_per_s": len(ok) / span, "p50_s": float(np.percentile(lat, 50)),
            "p99_s": float(np.percentile(lat, 99)), "mean_builds": float(np.mean([r["iters"] for r in ok])),
@@ -371,6 +394,125 @@ def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max
         out["max_abs_dE_vs_fp64_Eh"] = float(max(errs)) if errs else None
         out["dE_compared_jobs"] = len(errs)
     return out
+
+
+def big_leg(g, gbs, name, tau, steps, scf_iters, tune, purif_reps=0):
+    """The mixed-precision single-molecule workload: C54H18 / cc-pVDZ (846
+    basis functions).  FP64 and mixed Fock builds/s on the first SCF density
+    (tile shapes from the library's tuning tables for it when tune == "db"),
+    the purification of the first Fock matrix FP64 vs split BF16, and the SCF
+    FP64 vs mixed (FP32 ERIs below tau, split-BF16 purification) with the
+    energy difference of every iteration."""
+    from gtfock_amd import geometry, tuning
+    tmp = tempfile.mkdtemp()
+    xyz = os.path.join(tmp, name + ".xyz")
+    with open(xyz, "w") as fh:
+        fh.write(geometry.to_xyz(molecule(name)))
+    basis = g.BasisSet(gbs, xyz)
+    sh64 = shm = None
+    if tune == "db":
+        sh64, _ = tuning.load(name)
+        shm, _ = tuning.load(name + "_mixed")
+    pf = g.PFock(basis)
+    S, X, H = pf.overlap(), pf.x_matrix(), pf.core_hamiltonian()
+    D = first_density(g, pf, basis)
+    pf.put_den_mat(D)
+    pf.commit()
+    out = {"workload": "%s/cc-pVDZ" % name, "nbf": basis.nbf, "tau": tau,
+           "tile_shapes": "tuning tables %s, %s_mixed" % (name, name) if sh64 is not None else "built-in defaults"}
+
+    def timed(shapes):
+        if shapes is not None:
+            pf.set_tile_shapes(shapes)
+        pf.compute_fock()
+        device_sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            pf.compute_fock()
+        device_sync()
+        return steps / (time.perf_counter() - t0)
+
+    log("%s: %d timed FP64 builds" % (name, steps))
+    v64 = timed(sh64)
+    F64 = pf.get_mat(g.MAT_F)
+    pf.set_precision(tau)
+    log("%s: %d timed mixed builds" % (name, steps))
+    vm = timed(shm)
+    Fm = pf.get_mat(g.MAT_F)
+    nq_lo, _ = pf.mixed_counts()
+    nq, _ = pf.class_work()
+    pf.close()
+    out.update({"fp64_builds_per_s": v64, "mixed_builds_per_s": vm, "mixed_speedup": vm / v64,
+                "fp32_quartet_fraction": float(nq_lo.sum()) / max(1.0, float(nq.sum())),
+                "max_abs_dF_mixed": float(np.abs(Fm - F64).max()),
+                "first_order_dE_mixed_Eh": float(abs(np.sum(D * (Fm - F64))))})
+    # purification of the first Fock matrix: FP64 vs split BF16
+    Fo = X @ (H + F64) @ X
+    pur = {}
+    Ds = {}
+    for mode in ("fp64", "split"):
+        pu = g.Purif(basis)
+        pu.set_xs(X, S)
+        if mode != "fp64":
+            pu.set_precision(mode)
+        pu.purify(Fo)
+        t0 = time.perf_counter()
+        Dm, it = pu.purify(Fo)
+        dt = time.perf_counter() - t0
+        pur[mode] = {"seconds": dt, "iterations": it, "low_iterations": pu.fp32_iters()}
+        Ds[mode] = Dm
+        pu.close()
+    pur["split_speedup"] = pur["fp64"]["seconds"] / pur["split"]["seconds"]
+    pur["max_abs_dD"] = float(np.abs(Ds["split"] - Ds["fp64"]).max())
+    for mode in ("fp64", "split") if purif_reps > 0 else ():
+        # further purifications of the same matrix, for kernel profiles
+        pu = g.Purif(basis)
+        pu.set_xs(X, S)
+        if mode != "fp64":
+            pu.set_precision(mode)
+        for _ in range(purif_reps):
+            pu.purify(Fo)
+        pu.close()
+    out["purification"] = pur
+    if scf_iters > 0:
+        log("%s: SCF FP64" % name)
+        st64, stm = {}, {}
+        e64, _, _ = g.scf(gbs, xyz, scf_iters, ediff=1e-8, stats=st64, shapes=sh64)
+        log("%s: SCF mixed (FP32 ERIs below %g, split-BF16 purification)" % (name, tau))
+        em, Dm, _ = g.scf(gbs, xyz, scf_iters, ediff=1e-8, tau=tau, purif="split", stats=stm, shapes=shm)
+        n = min(len(e64), len(em))
+        d = np.abs(em[:n] - e64[:n])
+        tail = [k for k in range(1, n) if abs(e64[k] - e64[k - 1]) < 1e-6]
+        DS = Dm @ S
+        out["scf"] = {"ediff": 1e-8, "builds_fp64": len(e64), "builds_mixed": len(em),
+                      "E_fp64_Eh": float(e64[-1]), "E_mixed_Eh": float(em[-1]),
+                      "dE_final_Eh": float(abs(em[-1] - e64[-1])),
+                      "dE_per_iteration_Eh": [float(x) for x in d],
+                      "dE_converged_tail_max_Eh": float(max(d[k] for k in tail)) if tail else None,
+                      "idempotency_max_abs": float(np.abs(DS @ DS - DS).max()),
+                      "trace_DS_minus_nocc": float(np.trace(DS) - basis.nocc),
+                      "fock_s_fp64": st64.get("fock_s"), "fock_s_mixed": stm.get("fock_s"),
+                      "purif_s_fp64": st64.get("purif_s"), "purif_s_mixed": stm.get("purif_s"),
+                      "split_iterations_per_purification": stm.get("purif_low_iters")}
+    return out
+
+
+def copy_probe(mib, reps=10):
+    """Device-to-device copies of mib MiB by a plain elementwise kernel: each
+    reads and writes mib MiB, the yardstick that scales rocprofv3's
+    FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_MiB": mib, "reps": reps, "ms_per_copy": dt * 1e3,
+                      "GB_s_read_plus_write": 2 * mib * (1 << 20) / dt / 1e9}), flush=True)
 
 
 def load_traffic(workload, kernel):
@@ -407,10 +549,11 @@ def main():
                     help="directory to write the shapes tuned in this run to, as tuning tables")
     ap.add_argument("--batch-seed", type=int, default=20261202)
     ap.add_argument("--batch-scf", type=int, default=30, help="SCF iterations of the batched energy check (0: skip)")
-    ap.add_argument("--serve", type=int, default=16, help="jobs of the serving leg (0: skip)")
+    ap.add_argument("--serve", type=int, default=48, help="jobs of the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, jobs/s")
-    ap.add_argument("--serve-rings", type=int, default=3, help="largest graphene flake (rings) in the request stream")
-    ap.add_argument("--serve-budget", type=float, default=64.0, help="HBM budget of the server, GiB")
+    ap.add_argument("--serve-rings", type=int, default=4,
+                    help="largest graphene flake (rings) in the request stream (4: C96H24, 1464 basis functions)")
+    ap.add_argument("--serve-budget", type=float, default=128.0, help="HBM budget of the server, GiB")
     ap.add_argument("--serve-cohorts", type=int, default=16,
                     help="cohorts the server runs at once (16: a request starts as soon as it arrives at this load; "
                          "4 queued small requests behind flake cohorts, p50 9.6 s instead of 1.3-2.5 s)")
@@ -420,6 +563,20 @@ def main():
     ap.add_argument("--guess", choices=["atomic", "zero"], default="atomic",
                     help="starting density of the batched and served SCFs: scf.c's atomic blocks, generated by "
                          "gtfock_amd.guess, or zero")
+    ap.add_argument("--big", default="c54h18",
+                    help="molecule of the mixed-precision single-molecule leg ('none': skip)")
+    ap.add_argument("--big-steps", type=int, default=4, help="timed Fock builds per form of that leg")
+    ap.add_argument("--big-scf", type=int, default=40,
+                    help="SCF iterations (at most) of that leg's FP64 vs mixed comparison (0: skip)")
+    ap.add_argument("--purif-reps", type=int, default=0,
+                    help="extra purifications of that leg's first Fock matrix per mode (kernel profiles)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only time device copies of this many MiB (the reference kernel that calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--serve-purif", choices=["fp64", "fp32", "split"], default="split",
+                    help="purification products of the served SCFs")
+    ap.add_argument("--serve-join-gap", type=int, default=1,
+                    help="SCF iterations between admissions into running cohorts (< 0: off)")
     ap.add_argument("--scf-compare", type=int, default=0,
                     help="SCF iterations of a per-iteration FP64 vs mixed energy comparison on the workload molecule")
     args = ap.parse_args()
@@ -427,6 +584,9 @@ def main():
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(multi-GPU sharding is design-only)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
 
     import gtfock_amd as g
     from gtfock_amd import flops, geometry
@@ -488,11 +648,13 @@ def main():
                   (names[c], nq[c], npq[c], avg_s * 1e3, fl / avg_s / 1e12), file=sys.stderr)
     _, dom, dom_s, dom_fl = rows[0]
     achieved = dom_fl / dom_s
-    roofline = {"bound": "mfma", "peak_kind": "FP64 (vector and matrix peaks are both 78.6 TF/s on MI355X; "
-                "the ERI kernels run on the FP64 VALU)",
+    roofline = {"bound": "valu_fp64", "peak_kind": "FP64 VALU, 78.6 TF/s on MI355X (the ERI kernels run on the "
+                "vector ALUs, not on the matrix cores)",
                 "kernel": pf.class_kernel(dom), "class": names[dom],
                 "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK, "traffic": load_traffic(args.workload, pf.class_kernel(dom)),
+                "traffic_source": "copy-calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of this kernel "
+                                  "on this workload (profiles/eri_traffic_%s.json)" % args.workload,
                 "algorithmic_flops_per_launch": dom_fl, "avg_launch_ms": dom_s * 1e3,
                 "build_tflops": total_fl / (ms_step * 1e-3) / 1e12}
 
@@ -591,10 +753,16 @@ def main():
                     ranges.append((lo, hi, sh))
                     used.append("%s for nbf %d.." % (name, lo))
         served = serve_leg(g, scf_gbs, args.serve, args.serve_rate, 20261203, args.tau, args.serve_rings,
-                           args.serve_budget, args.serve_cohorts, 40, True, ranges, args.serve_large)
+                           args.serve_budget, args.serve_cohorts, 40, True, ranges, args.serve_large,
+                           args.serve_purif, args.serve_join_gap)
         served["large_slots"] = args.serve_large
         served["initial_guess"] = args.guess
         served["tile_shapes"] = ("tuning tables: " + ", ".join(used)) if used else "built-in defaults"
+
+    big = None
+    if args.big != "none":
+        log("mixed-precision leg: %s" % args.big)
+        big = big_leg(g, gbs, args.big, args.tau, args.big_steps, args.big_scf, args.tune, args.purif_reps)
 
     batched = None
     if args.batch > 0:
@@ -605,7 +773,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log("CPU baseline: one oracle Fock build")
+        log("CPU baseline: oracle Fock builds")
         cpu = cpu_baseline(gbs, xyz, D, args.tol)
 
     st = pf.stats()
@@ -620,6 +788,7 @@ def main():
         "roofline": roofline,
         "tile_shapes": tune,
         "mixed": mixed,
+        "mixed_c54h18": big,
         "batched": batched,
         "serve": served,
         "scf_compare": scf_cmp,

@@ -104,6 +104,8 @@ def lib():
         "PFock_getMemorySize": ([_vp, _dp], ctypes.c_int),
         "PFock_getStatistics": ([_vp], ctypes.c_int),
         "gtf_pfock_stats": ([_vp, _dp], ctypes.c_int),
+        "gtf_pfock_fused_densities": ([_vp], ctypes.c_int),
+        "gtf_pfock_balance": ([_vp, _dp], ctypes.c_int),
         "gtf_pfock_csr": ([_vp, _ip, _ip, _ip, _dp], ctypes.c_int),
         "gtf_pfock_class_counts": ([_vp, _lp], ctypes.c_int),
         "gtf_pfock_quartets": ([_vp, _ip, ctypes.c_long], ctypes.c_long),
@@ -142,6 +144,9 @@ def lib():
         "gtf_purif_set_precision": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "gtf_purif_fp32_iters": ([_vp], ctypes.c_int),
         "gtf_purif_set_slices": ([_vp, ctypes.c_int], ctypes.c_int),
+        "gtf_purif_set_split_residual": ([_vp, ctypes.c_double], ctypes.c_int),
+        "gtf_batch_set_purif_residual": ([_vp, ctypes.c_double], ctypes.c_int),
+        "gtf_tile_shapes_check": ([_ip], ctypes.c_int),
         "gtf_gemm_split_host": ([ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "gtf_batch_set_purif_precision": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "gtf_batch_set_purif_slices": ([_vp, ctypes.c_int], ctypes.c_int),
@@ -174,12 +179,21 @@ def lib():
         "gtf_server_stats": ([_vp, _dp], ctypes.c_int),
         "gtf_server_destroy": ([_vp], None),
     }
+    # a library older than this binding lacks some entry points: only calls
+    # to those fail (AttributeError); tests/test_host.py checks that the
+    # built library exports every symbol the headers declare
     for name, (args, res) in sig.items():
+        if not hasattr(L, name):
+            MISSING_SYMBOLS.append(name)
+            continue
         f = getattr(L, name)
         f.argtypes = args
         f.restype = res
     _lib = L
     return L
+
+
+MISSING_SYMBOLS = []
 
 
 def exported_symbols():
@@ -302,6 +316,14 @@ class PFock:
         lib().gtf_pfock_stats(self.h, _p(out))
         keys = ["usq", "uitl", "nnz", "maxvalue", "gpu_seconds", "wall_seconds", "create_seconds", "device_bytes"]
         return dict(zip(keys, out.tolist()))
+
+    def balance(self):
+        """Load balance of the last build: slowest persistent workgroup over
+        the mean (summed over the tile launches), tiles taken after each
+        workgroup's first, workgroups."""
+        out = np.zeros(3)
+        _check(lib().gtf_pfock_balance(self.h, _p(out)), "gtf_pfock_balance")
+        return dict(load_balance=out[0], steals=int(out[1]), workgroups=int(out[2]))
 
     def csr(self):
         ns = self.basis.nshells
@@ -507,6 +529,11 @@ class Purif:
         """Slices per operand of the split-BF16 products (3..6, default 5)."""
         _check(lib().gtf_purif_set_slices(self.h, int(ns)), "gtf_purif_set_slices")
 
+    def set_split_residual(self, resid):
+        """Split products only while the SCF's last DIIS residual is at least
+        resid (gtf_purif_set_split_residual; 0: in every SCF iteration)."""
+        _check(lib().gtf_purif_set_split_residual(self.h, float(resid)), "gtf_purif_set_split_residual")
+
     def fp32_iters(self):
         """iterations of the last purification run in the reduced precision"""
         return lib().gtf_purif_fp32_iters(self.h)
@@ -572,6 +599,10 @@ class Batch:
 
     def set_purif_slices(self, ns):
         self._st(lib().gtf_batch_set_purif_slices(self.h, int(ns)), "gtf_batch_set_purif_slices")
+
+    def set_purif_residual(self, resid):
+        """as Purif.set_split_residual, per molecule of the batch"""
+        self._st(lib().gtf_batch_set_purif_residual(self.h, float(resid)), "gtf_batch_set_purif_residual")
 
     def autotune(self, reps=3):
         """Tile autotuner on the batch's density (gtf_batch_autotune)."""
@@ -835,25 +866,35 @@ def initial_guess(basis):
 
 
 def scf(gbs, xyz, niters, tolscr=1e-11, verbose=False, tau=0.0, purif_fp32=False, guess=False, purif=None,
-        ediff=1e-11, stats=None):
+        ediff=1e-11, stats=None, shapes=None, purif_ns=None, purif_resid=None):
     """pscf/scf.c:189-474 on one GPU.  The initial density is zero
     (core-Hamiltonian guess), or with guess=True the atomic-block guess of
     scf.c's initial_guess (zero for atoms without a .dat file).  tau > 0
     selects the mixed-precision Fock build; purif ("fp64", "fp32", "split";
-    purif_fp32=True is "fp32") the purification products.  The SCF stops
+    purif_fp32=True is "fp32") the purification products, purif_ns and
+    purif_resid the split products' slices and DIIS-residual switch (None:
+    the library defaults).  The SCF stops
     after the build whose energy differs from the previous one by less than
     ediff (scf.c: 1e-11).  stats (a dict, may be None) receives the
-    purification iteration counts.  Returns (energies, last D, last F =
-    H + G(D))."""
+    purification iteration counts and the seconds spent in Fock builds and
+    in purifications; shapes: tile shapes of the Fock builds (None: the
+    built-in defaults).  Returns (energies, last D, last F = H + G(D))."""
+    import time
     basis = BasisSet(gbs, xyz)
     pf = PFock(basis, tolscr=tolscr)
     if tau > 0:
         pf.set_precision(tau)
+    if shapes is not None:
+        pf.set_tile_shapes(shapes)
     pu = Purif(basis)
     if purif is None:
         purif = "fp32" if purif_fp32 else "fp64"
     if purif != "fp64":
         pu.set_precision(purif)
+    if purif_ns is not None:
+        pu.set_slices(purif_ns)
+    if purif_resid is not None:
+        pu.set_split_residual(purif_resid)
     S = pf.overlap()
     X = pf.x_matrix()
     H = pf.core_hamiltonian()
@@ -866,7 +907,10 @@ def scf(gbs, xyz, niters, tolscr=1e-11, verbose=False, tau=0.0, purif_fp32=False
     F = None
     Dlast = D
     for it in range(niters):
+        t0 = time.perf_counter()
         G = pf.fock(D)
+        if stats is not None:
+            stats["fock_s"] = stats.get("fock_s", 0.0) + time.perf_counter() - t0
         F = G + H
         e = float(np.sum(D * (H + F)))
         energies.append(e + enuc)
@@ -877,8 +921,10 @@ def scf(gbs, xyz, niters, tolscr=1e-11, verbose=False, tau=0.0, purif_fp32=False
             break
         e0 = e
         Fo = pu.diis(pf, D, F, it)
+        t0 = time.perf_counter()
         D, pit = pu.purify(Fo)
         if stats is not None:
+            stats["purif_s"] = stats.get("purif_s", 0.0) + time.perf_counter() - t0
             stats.setdefault("purif_iters", []).append(pit)
             stats.setdefault("purif_low_iters", []).append(pu.fp32_iters())
     pu.close()

@@ -89,14 +89,22 @@ struct GtfBatch {
     // purif_ns slices (ozaki.h); d_low[m] holds the mode a molecule is in
     int purif_mode = 0;
     double purif_until = 1e-3;
-    int purif_ns = 5;
+    int purif_ns = OZ_SLICES_DEFAULT;
+    // the split products run only while a molecule's last DIIS residual
+    // ||X (F D S - S D F) X||_F exceeds purif_resid (before its first: split);
+    // closer to convergence its purification is FP64 throughout
+    double purif_resid = OZ_SPLIT_RESID;
+    std::vector<double> resid;  // per molecule, -1 before the first
+    std::vector<int> h_low;     // source of the copy into d_low
     int* d_low = nullptr;
     void* oz_mem = nullptr;
     OzScratch oz;
     BatchLayout layout() const { return BatchLayout{nmol, maxn, maxpld, d_off, d_n, d_poff, d_pld, d_part, d_pcoef}; }
     int enqueue_fock();
     int fock();
-    int scf_setup();
+    // from / from_idx (may be null): molecule m's one-electron matrices and
+    // X are copied from molecule from_idx[m] of batch `from` when >= 0
+    int scf_setup(const GtfBatch* from = nullptr, const int* from_idx = nullptr);
     int put_initial_guess();
     int diis(const std::vector<int>& iter, const std::vector<int>& active);
     // step-wise SCF (gtf_batch_scf_begin / _step) and its per-molecule state
@@ -235,9 +243,10 @@ int GtfBatch::put_initial_guess()
     return 0;
 }
 
-int GtfBatch::scf_setup()
+int GtfBatch::scf_setup(const GtfBatch* from, const int* from_idx)
 {
     if (scf_ready) return 0;
+    auto carried = [&](int m) { return from && from_idx && from_idx[m] >= 0; };
     GtfSystem& s = sys;
     d_H = s.dalloc<double>(total);
     d_S = s.dalloc<double>(total);
@@ -272,6 +281,7 @@ int GtfBatch::scf_setup()
         return -1;
     // one-electron matrices per molecule over its slice of the shell arrays
     for (int m = 0; m < nmol; m++) {
+        if (carried(m)) continue;
         const int s0 = s.mol_shell_off[m], a0 = atom_off[m];
         BasisView bv{s.mol_nshells[m], n[m], bases[m]->natoms, s.d_L + s0, s.d_nprim + s0, s.d_poff + s0,
                      s.d_fstart + s0, s.d_atom + s0, s.d_alpha, s.d_coef, s.d_center + 3 * s0,
@@ -279,37 +289,75 @@ int GtfBatch::scf_setup()
         if (gtf_oneel(stream, bv, s.d_boys, n[m], d_S + off[m], T + off[m], V + off[m])) return -1;
     }
     if (gtf_add(stream, T, V, d_H, total)) return -1;
-    // X = S^-1/2 per molecule
-    double* W = s.dalloc<double>((size_t)maxn * maxn);
-    double* lam = s.dalloc<double>(maxn);
-    double* work = s.dalloc<double>(maxn);
-    int* info = s.dalloc<int>(1);
+    for (int m = 0; m < nmol; m++) {
+        if (!carried(m)) continue;
+        const int mo = from_idx[m];
+        const size_t nn = (size_t)n[m] * n[m];
+        if (hipMemcpyAsync(d_H + off[m], from->d_H + from->off[mo], sizeof(double) * nn, hipMemcpyDeviceToDevice,
+                           stream) != hipSuccess ||
+            hipMemcpyAsync(d_S + off[m], from->d_S + from->off[mo], sizeof(double) * nn, hipMemcpyDeviceToDevice,
+                           stream) != hipSuccess ||
+            hipMemcpyAsync(p_X + poff[m], from->p_X + from->poff[mo], sizeof(double) * pld[m] * pld[m],
+                           hipMemcpyDeviceToDevice, stream) != hipSuccess)
+            return -1;
+    }
+    // X = S^-1/2 per molecule: molecules of one basis size share one
+    // strided-batched eigensolver call, the return codes are read once
+    std::vector<std::vector<int>> groups;
+    {
+        std::vector<int> order(nmol);
+        for (int m = 0; m < nmol; m++) order[m] = m;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] < n[y]; });
+        int prev = -1;
+        for (int k = 0; k < nmol; k++) {
+            if (carried(order[k])) continue;
+            if (prev < 0 || n[order[k]] != n[prev]) groups.emplace_back();
+            groups.back().push_back(order[k]);
+            prev = order[k];
+        }
+    }
+    size_t wmax = 1;
+    for (const auto& gr : groups) wmax = std::max(wmax, gr.size() * (size_t)n[gr[0]] * n[gr[0]]);
+    double* W = s.dalloc<double>(wmax);
+    double* lam = s.dalloc<double>((size_t)nmol * maxn);
+    double* work = s.dalloc<double>((size_t)nmol * maxn);
+    int* info = s.dalloc<int>(nmol);
     if (!W || !lam || !work || !info) return -2;
     rocblas_handle h;
     if (rocblas_create_handle(&h) != rocblas_status_success) return -1;
     rocblas_set_stream(h, stream);
-    int bad = 0;
-    for (int m = 0; m < nmol && !bad; m++) {
-        const int nm = n[m];
-        if (hipMemcpyAsync(W, d_S + off[m], sizeof(double) * nm * nm, hipMemcpyDeviceToDevice, stream) != hipSuccess) {
-            bad = 1;
-            break;
-        }
-        if (rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, nm, W, nm, lam, work, info) !=
-            rocblas_status_success) {
+    int bad = 0, done = 0;
+    for (const auto& gr : groups) {
+        const int nm = n[gr[0]], cnt = (int)gr.size();
+        const size_t nn = (size_t)nm * nm;
+        // the group's results go to lam / info slots done .. done + cnt - 1
+        for (int k = 0; k < cnt && !bad; k++)
+            if (hipMemcpyAsync(W + k * nn, d_S + off[gr[k]], sizeof(double) * nn, hipMemcpyDeviceToDevice, stream) !=
+                hipSuccess)
+                bad = 1;
+        if (bad) break;
+        if (rocsolver_dsyevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, nm, W, nm, (rocblas_stride)nn,
+                                             lam + (size_t)done * maxn, maxn, work + (size_t)done * maxn, maxn,
+                                             info + done, cnt) != rocblas_status_success) {
             bad = 2;
             break;
         }
-        int hinfo = 0;
-        if (hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, stream) != hipSuccess ||
-            hipStreamSynchronize(stream) != hipSuccess || hinfo != 0) {
-            bad = 3;
-            break;
+        for (int k = 0; k < cnt; k++) {
+            const int m = gr[k];
+            dim3 g((pld[m] + 15) / 16, (pld[m] + 15) / 16), b(16, 16);
+            binv_sqrt_kernel<<<g, b, 0, stream>>>(W + k * nn, lam + (size_t)(done + k) * maxn, nm, pld[m],
+                                                  p_X + poff[m]);
+            if (hipGetLastError() != hipSuccess) bad = 4;
         }
-        dim3 g((pld[m] + 15) / 16, (pld[m] + 15) / 16), b(16, 16);
-        binv_sqrt_kernel<<<g, b, 0, stream>>>(W, lam, nm, pld[m], p_X + poff[m]);
-        if (hipGetLastError() != hipSuccess) bad = 4;
+        done += cnt;
     }
+    std::vector<int> hinfo(nmol, 0);
+    if (!bad && done > 0 &&
+        (hipMemcpyAsync(hinfo.data(), info, sizeof(int) * done, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+         hipStreamSynchronize(stream) != hipSuccess))
+        bad = 3;
+    for (int k = 0; k < done && !bad; k++)
+        if (hinfo[k] != 0) bad = 3;
     rocblas_destroy_handle(h);
     if (bad) {
         GTF_PRINTF("S^-1/2 failed (%d)\n", bad);
@@ -377,6 +425,7 @@ int GtfBatch::scf_begin(int max_iters, double ediff, gtf_scf_done_fn done, void*
     len_diis.assign(nmol, 0);
     bmax_id.assign(nmol, -1);
     bmax.assign(nmol, DBL_MIN);
+    resid.assign(nmol, -1.0);
     b_mat.assign((size_t)nmol * LB * LB, 0.0);
     for (int m = 0; m < nmol; m++)
         for (int i = 0; i <= B_MAX_DIIS; i++)
@@ -528,6 +577,7 @@ int GtfBatch::diis(const std::vector<int>& iter, const std::vector<int>& active)
                     bmax[m] = bm[i * LB + i];
                     bmax_id[m] = i;
                 }
+            resid[m] = std::sqrt(std::max(bm[cur * LB + cur], 0.0));
         }
     }
     // F_orth = X F X
@@ -559,7 +609,11 @@ int GtfBatch::purify(const std::vector<int>& active)
     const int* skip = d_skip;
     if (gtf_bpurif_init(stream, L, d_nocc, skip, p_F, p_D)) return -1;
     if (gtf_fill_int(stream, d_iters, nmol, -1)) return -1;
-    if (gtf_fill_int(stream, d_low, nmol, purif_mode)) return -1;
+    h_low.assign(nmol, purif_mode);
+    if (purif_mode == GTF_LOW_SPLIT)
+        for (int m = 0; m < nmol; m++)
+            if (resid[m] >= 0.0 && resid[m] < purif_resid) h_low[m] = GTF_LOW_FP64;
+    B_CHECK(hipMemcpyAsync(d_low, h_low.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, stream));
     if (purif_mode == GTF_LOW_SPLIT && !oz_mem) {
         oz_mem = sys.dalloc<char>(gtf_oz_scratch_bytes(ptotal, OZ_SLICES_MAX));
         if (!oz_mem) return -2;
@@ -636,6 +690,7 @@ int gtf_batch_create(BasisSet_t* basis, int nmol, double tolscr, gtf_batch_t* ou
     b->len_diis.assign(nmol, 0);
     b->bmax_id.assign(nmol, -1);
     b->bmax.assign(nmol, DBL_MIN);
+    b->resid.assign(nmol, -1.0);
     b->purif_iters.assign(nmol, 0);
     b->done_time.assign(nmol, 0.0);
     if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
@@ -750,6 +805,13 @@ int gtf_batch_set_purif_precision(gtf_batch_t b, int mode, double until)
     return 0;
 }
 
+int gtf_batch_set_purif_residual(gtf_batch_t b, double resid)
+{
+    if (!b || !(resid >= 0.0)) return -3;
+    b->purif_resid = resid;
+    return 0;
+}
+
 int gtf_batch_set_purif_slices(gtf_batch_t b, int ns)
 {
     if (!b || ns < OZ_SLICES_MIN || ns > OZ_SLICES_MAX) return -3;
@@ -856,7 +918,7 @@ int gtf_batch_scf_cb(gtf_batch_t b, int max_iters, double ediff, double* energy,
     }
     if (rc == 0) rc = b->scf_finish();
     if (rc) return rc;
-    return gtf_batch_scf_results(b, energy, iters, history);
+    return gtf_batch_scf_results(b, energy, iters, history) < 0 ? -1 : 0;
 }
 
 int gtf_batch_scf_begin(gtf_batch_t b, int max_iters, double ediff, gtf_scf_done_fn done, void* ctx)
@@ -911,6 +973,11 @@ int gtf_batch_regroup(gtf_batch_t b, BasisSet_t* add, int nadd, gtf_batch_t* out
     nb->purif_mode = b->purif_mode;
     nb->purif_until = b->purif_until;
     nb->purif_ns = b->purif_ns;
+    nb->purif_resid = b->purif_resid;
+    // the carried molecules' one-electron matrices and X come from the old batch
+    std::vector<int> from(nnew, -1);
+    for (int k = 0; k < nkeep; k++) from[k] = keep[k];
+    if (rc == 0 && b->scf_ready) rc = nb->scf_setup(b, from.data());
     if (rc == 0) rc = nb->scf_begin(b->run_max_iters, b->run_ediff, b->run_done, b->run_ctx);
     if (rc) {
         delete nb;
@@ -938,6 +1005,7 @@ int gtf_batch_regroup(gtf_batch_t b, BasisSet_t* add, int nadd, gtf_batch_t* out
         nb->len_diis[k] = b->len_diis[mo];
         nb->bmax_id[k] = b->bmax_id[mo];
         nb->bmax[k] = b->bmax[mo];
+        nb->resid[k] = b->resid[mo];
         std::copy(b->b_mat.begin() + (size_t)mo * LB * LB, b->b_mat.begin() + (size_t)(mo + 1) * LB * LB,
                   nb->b_mat.begin() + (size_t)k * LB * LB);
         nb->run_next[k] = b->run_next[mo];
@@ -983,7 +1051,10 @@ double gtf_batch_job_bytes(BasisSet_t basis)
             pairs += np * (32.0 + 8.0 * (nsph(la) * nsph(lb) * nherm(la + lb) + herm_nnz(la, lb))) + 96.0;
         }
     const double n = b->nbf, pl = ((b->nbf + 63) / 64) * 64;
-    return pairs * 1.25 + 8.0 * (9.0 * n * n + (8.0 + 2.0 * B_MAX_DIIS) * pl * pl) + (1 << 20);
+    // + the split purification's BF16 slices of both operands at the most
+    // slices (and their exponents and row maxima)
+    const double split = 2.0 * 2.0 * OZ_SLICES_MAX * pl * pl + 4.0 * 8.0 * pl * pl / 64.0;
+    return pairs * 1.25 + 8.0 * (9.0 * n * n + (8.0 + 2.0 * B_MAX_DIIS) * pl * pl) + split + (1 << 20);
 }
 
 int gtf_batch_get_density(gtf_batch_t b, int m, double* D)

@@ -31,9 +31,11 @@ int gtf_bpad(hipStream_t st, const BatchLayout& L, const int* skip, const double
 int gtf_bunpad(hipStream_t st, const BatchLayout& L, const int* skip, const double* padded, double* packed);
 int gtf_bpurif_init(hipStream_t st, const BatchLayout& L, const int* nocc, const int* skip, const double* F, double* D);
 // also sets fp32[m] (may be null) to GTF_LOW_FP64 once ||D - D^2||_F < fp32_until
+// and records in low_iters[m] (may be null) how many iterations ran in the
+// reduced precision
 int gtf_bpurif_stats(hipStream_t st, const BatchLayout& L, const int* skip, const double* D, const double* D2,
                      const double* D3, int* iters, double* stats, int it, double tol, int* fp32 = nullptr,
-                     double fp32_until = 0.0);
+                     double fp32_until = 0.0, int* low_iters = nullptr);
 int gtf_bpurif_update(hipStream_t st, const BatchLayout& L, const int* skip, const int* iters, const double* stats,
                       double* D, const double* D2, const double* D3, int it);
 int gtf_bantisym(hipStream_t st, const BatchLayout& L, const int* skip, const double* W, double* E);

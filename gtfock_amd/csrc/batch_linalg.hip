@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void bstats_part_kernel(const long* __restrict
 // iteration's update
 __global__ void bstats_fin_kernel(int nmol, const int* __restrict__ skip, const double* __restrict__ part,
                                   int* __restrict__ iters, double* __restrict__ stats, int it, double tol,
-                                  int* __restrict__ fp32, double fp32_until)
+                                  int* __restrict__ fp32, double fp32_until, int* __restrict__ low_iters)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nmol || (skip && skip[m]) || iters[m] >= 0) return;
@@ -216,7 +216,10 @@ __global__ void bstats_fin_kernel(int nmol, const int* __restrict__ skip, const 
     stats[3 * m + 2] = t2;
     if (sqrt(e) < tol) iters[m] = it;
     // FP32 GEMMs only while the error is far above FP32 resolution
-    if (fp32 && sqrt(e) < fp32_until) fp32[m] = 0;
+    if (fp32 && fp32[m] && sqrt(e) < fp32_until) {
+        fp32[m] = 0;
+        if (low_iters) low_iters[m] = it + 1;  // iterations 0..it ran in the reduced precision
+    }
 }
 
 // D <- ((1-2c) D + (1+c) D2 - D3)/(1-c) if c < 1/2, else ((1+c) D2 - D3)/c
@@ -385,12 +388,13 @@ int gtf_bpurif_init(hipStream_t st, const BatchLayout& L, const int* nocc, const
 }
 
 int gtf_bpurif_stats(hipStream_t st, const BatchLayout& L, const int* skip, const double* D, const double* D2,
-                     const double* D3, int* iters, double* stats, int it, double tol, int* fp32, double fp32_until)
+                     const double* D3, int* iters, double* stats, int it, double tol, int* fp32, double fp32_until,
+                     int* low_iters)
 {
     bstats_part_kernel<<<dim3(BPART, L.nmol), 256, 0, st>>>(L.poff, L.n, L.pld, skip, iters, D, D2, D3, L.part);
     GTF_CHECK(hipGetLastError());
     bstats_fin_kernel<<<(L.nmol + 63) / 64, 64, 0, st>>>(L.nmol, skip, L.part, iters, stats, it, tol, fp32,
-                                                          fp32_until);
+                                                          fp32_until, low_iters);
     GTF_CHECK(hipGetLastError());
     return 0;
 }

@@ -255,7 +255,6 @@ int GtfSystem::screen(double tolscr)
     std::vector<unsigned long long> vb(npairs);
     SYS_CHECK(hipMemcpyAsync(vb.data(), d_vbits, sizeof(unsigned long long) * npairs, hipMemcpyDeviceToHost, stream));
     SYS_CHECK(hipStreamSynchronize(stream));
-    dfree(d_vbits);
     dfree(d_q);
     dfree(d_ids);
     h_v.resize(npairs);
@@ -268,82 +267,100 @@ int GtfSystem::screen(double tolscr)
     }
     vmax = mol_vmax[0];
     // significant pairs of molecule m: v > tol^2 / vmax_m (pfock/screening.c:111,129),
-    // in one segment per (molecule, primitive-count bucket)
+    // in one segment per (molecule, primitive-count bucket), compacted and
+    // ordered on the device (sigsort.hip); the host keeps copies for the
+    // tile cutting
     std::vector<double> eta(nmol);
     for (int m = 0; m < nmol; m++) eta[m] = mol_vmax[m] > 0.0 ? tol2 / mol_vmax[m] : HUGE_VAL;
     nseg = nmol * NBUCKET;
+    double* d_eta = dalloc<double>(nmol);
+    int* d_ptype = dalloc<int>(npairs);
+    int* d_pmol = dalloc<int>(npairs);
+    const int nblocks = (npairs + 255) / 256;
+    int* d_bcount = dalloc<int>(nblocks);
+    int* d_boff = dalloc<int>(nblocks);
+    int* d_total = dalloc<int>(1);
+    if (!d_eta || !d_ptype || !d_pmol || !d_bcount || !d_boff || !d_total) return -2;
+    if (upload(d_eta, eta, stream) || upload(d_ptype, h_type, stream) || upload(d_pmol, h_pair_mol, stream)) return -1;
     for (int t = 0; t < GTF_NPAIRTYPES; t++) {
-        std::vector<int>& s = h_sig[t];
-        s.clear();
         h_listprim[t].clear();
         h_segmax[t].clear();
-        h_sigv[t].clear();
-        h_rank[t].clear();
-        h_sigmol[t].clear();
-        h_sorted[t].clear();
-        h_sortrank[t].clear();
-        h_sortnprim[t].clear();
-        seg_start[t].assign(nseg, 0);
-        seg_len[t].assign(nseg, 0);
-        // bytype[t] is ordered by pair id, hence grouped by molecule
-        size_t i0 = 0;
-        for (int m = 0; m < nmol; m++) {
-            std::vector<int> mol;
-            while (i0 < bytype[t].size() && h_pair_mol[bytype[t][i0]] == m) {
-                const int id = bytype[t][i0++];
-                if (h_v[id] > eta[m]) mol.push_back(id);
-            }
-            std::sort(mol.begin(), mol.end(),
-                      [&](int x, int y) { return h_v[x] != h_v[y] ? h_v[x] > h_v[y] : x < y; });
-            std::vector<int> rank_of(mol.size());
-            std::iota(rank_of.begin(), rank_of.end(), 0);
-            for (int b = 0; b < NBUCKET; b++) {
-                // the bucket's pairs in value order (ranks rising)
-                std::vector<int> seg, segrank;
-                for (size_t k = 0; k < mol.size(); k++)
-                    if (bucket(h_pnprim[mol[k]]) == b) {
-                        seg.push_back(mol[k]);
-                        segrank.push_back((int)k);
-                    }
-                // each block re-sorted by primitive count (descending)
-                std::vector<int> order(seg.size());
-                std::iota(order.begin(), order.end(), 0);
-                for (size_t b0 = 0; b0 < seg.size(); b0 += GTF_KET_BLOCK) {
-                    const size_t b1 = std::min(seg.size(), b0 + GTF_KET_BLOCK);
-                    std::stable_sort(order.begin() + b0, order.begin() + b1,
-                                     [&](int x, int y) { return h_pnprim[seg[x]] > h_pnprim[seg[y]]; });
-                }
-                const int sg = m * NBUCKET + b;
-                seg_start[t][sg] = (int)s.size();
-                seg_len[t][sg] = (int)seg.size();
-                for (size_t k = 0; k < seg.size(); k++) {
-                    s.push_back(seg[order[k]]);
-                    h_rank[t].push_back(segrank[order[k]]);
-                    h_sigv[t].push_back(h_v[seg[order[k]]]);
-                    h_sigmol[t].push_back(m);
-                    h_sorted[t].push_back(h_v[seg[k]]);
-                    h_sortrank[t].push_back(segrank[k]);
-                    h_sortnprim[t].push_back(h_pnprim[seg[k]]);
-                }
-            }
-        }
-        nsig[t] = (int)s.size();
-        d_sig[t] = dalloc<int>(s.size());
-        d_sigv[t] = dalloc<double>(s.size());
-        d_rank[t] = dalloc<int>(s.size());
-        d_sigv_sorted[t] = dalloc<double>(s.size());
-        d_sortrank[t] = dalloc<int>(s.size());
-        d_seg_start[t] = dalloc<int>(nseg);
-        d_seg_len[t] = dalloc<int>(nseg);
-        if (!d_sig[t] || !d_sigv[t] || !d_rank[t] || !d_sigv_sorted[t] || !d_sortrank[t] || !d_seg_start[t] ||
-            !d_seg_len[t])
+        int n = 0;
+        if (gtf_sig_count(stream, npairs, d_ptype, t, d_vbits, d_pmol, d_eta, d_bcount, d_boff, d_total)) return -1;
+        SYS_CHECK(hipMemcpyAsync(&n, d_total, sizeof(int), hipMemcpyDeviceToHost, stream));
+        SYS_CHECK(hipStreamSynchronize(stream));
+        nsig[t] = n;
+        const size_t nn = std::max(n, 1);
+        SigWork w{};
+        SigOut o{};
+        w.ids_a = dalloc<int>(nn);
+        w.ids_b = dalloc<int>(nn);
+        w.pos_a = dalloc<int>(nn);
+        w.pos_b = dalloc<int>(nn);
+        w.rank = dalloc<int>(nn);
+        w.k64_a = dalloc<unsigned long long>(nn);
+        w.k64_b = dalloc<unsigned long long>(nn);
+        w.k32_a = dalloc<unsigned>(nn);
+        w.k32_b = dalloc<unsigned>(nn);
+        w.temp_bytes = gtf_sig_temp_bytes((int)nn);
+        w.temp = dalloc<char>(w.temp_bytes);
+        o.s_id = dalloc<int>(nn);
+        o.sortnprim = dalloc<int>(nn);
+        o.sigmol = dalloc<int>(nn);
+        d_sig[t] = o.sig = dalloc<int>(nn);
+        d_sigv[t] = o.sigv = dalloc<double>(nn);
+        d_rank[t] = o.rank = dalloc<int>(nn);
+        d_sigv_sorted[t] = o.sorted = dalloc<double>(nn);
+        d_sortrank[t] = o.sortrank = dalloc<int>(nn);
+        d_seg_start[t] = o.seg_start = dalloc<int>(nseg);
+        d_seg_len[t] = o.seg_len = dalloc<int>(nseg);
+        if (!w.ids_a || !w.ids_b || !w.pos_a || !w.pos_b || !w.rank || !w.k64_a || !w.k64_b || !w.k32_a ||
+            !w.k32_b || !w.temp || !o.s_id || !o.sortnprim || !o.sigmol || !o.sig || !o.sigv || !o.rank ||
+            !o.sorted || !o.sortrank || !o.seg_start || !o.seg_len)
             return -2;
-        if (upload(d_sig[t], s, stream) || upload(d_sigv[t], h_sigv[t], stream) ||
-            upload(d_rank[t], h_rank[t], stream) || upload(d_sigv_sorted[t], h_sorted[t], stream) ||
-            upload(d_sortrank[t], h_sortrank[t], stream) || upload(d_seg_start[t], seg_start[t], stream) ||
-            upload(d_seg_len[t], seg_len[t], stream))
+        if (gtf_fill_int(stream, o.seg_start, nseg, 0) || gtf_fill_int(stream, o.seg_len, nseg, 0)) return -1;
+        if (gtf_sig_order(stream, npairs, d_ptype, t, d_vbits, d_pmol, d_eta, d_pnprim, NBUCKET, d_boff, n, w, o))
             return -1;
+        h_sig[t].resize(n);
+        h_sigv[t].resize(n);
+        h_rank[t].resize(n);
+        h_sigmol[t].resize(n);
+        h_sorted[t].resize(n);
+        h_sortrank[t].resize(n);
+        h_sortnprim[t].resize(n);
+        seg_len[t].assign(nseg, 0);
+        seg_start[t].assign(nseg, 0);
+        if (n > 0) {
+            SYS_CHECK(hipMemcpyAsync(h_sig[t].data(), o.sig, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+            SYS_CHECK(hipMemcpyAsync(h_sigv[t].data(), o.sigv, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
+            SYS_CHECK(hipMemcpyAsync(h_rank[t].data(), o.rank, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+            SYS_CHECK(hipMemcpyAsync(h_sigmol[t].data(), o.sigmol, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+            SYS_CHECK(hipMemcpyAsync(h_sorted[t].data(), o.sorted, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
+            SYS_CHECK(hipMemcpyAsync(h_sortrank[t].data(), o.sortrank, sizeof(int) * n, hipMemcpyDeviceToHost,
+                                     stream));
+            SYS_CHECK(hipMemcpyAsync(h_sortnprim[t].data(), o.sortnprim, sizeof(int) * n, hipMemcpyDeviceToHost,
+                                     stream));
+            SYS_CHECK(hipMemcpyAsync(seg_len[t].data(), o.seg_len, sizeof(int) * nseg, hipMemcpyDeviceToHost, stream));
+        }
+        SYS_CHECK(hipStreamSynchronize(stream));
+        // segment starts in segment order, empty segments included
+        for (int g = 0, at = 0; g < nseg; g++) {
+            seg_start[t][g] = at;
+            at += seg_len[t][g];
+        }
+        if (upload(o.seg_start, seg_start[t], stream)) return -1;
+        SYS_CHECK(hipStreamSynchronize(stream));
+        void* scratch[] = {w.ids_a, w.ids_b, w.pos_a, w.pos_b, w.rank, w.k64_a, w.k64_b, w.k32_a, w.k32_b, w.temp,
+                           o.s_id, o.sortnprim, o.sigmol};
+        for (void* q : scratch) dfree(q);
     }
+    dfree(d_vbits);
+    dfree(d_eta);
+    dfree(d_ptype);
+    dfree(d_pmol);
+    dfree(d_bcount);
+    dfree(d_boff);
+    dfree(d_total);
     // quartet rows per class: bra type A >= ket type B; a row pairs one bra
     // with one non-empty ket segment of its molecule
     int maxrows = 1;
@@ -523,8 +540,10 @@ int GtfSystem::build_class_tiles(int cls)
         // kets a tile can hold (its J_PQ rows live in LDS)
         const int ns = exchanged(cls) ? ci->nsplit_t : ci->nsplit;
         const int kmax = std::max(1, sh.units / ns);
-        tile_lds[cls] = sizeof(double) * ((sh.boys_lds ? GTF_BOYS_LDS : 0) + (size_t)(NA + NB) * nbf + NA * NB +
-                                          (size_t)kmax * NCD);
+        // LDS of a launch: the Boys slab (if staged) plus one work area
+        // (strips, J_MN, J_PQ rows) per density digested
+        tile_lds_work[cls] = sizeof(double) * ((size_t)(NA + NB) * nbf + NA * NB + (size_t)kmax * NCD);
+        tile_lds[cls] = sizeof(double) * (sh.boys_lds ? GTF_BOYS_LDS : 0) + tile_lds_work[cls];
         ntiles[cls] = ntiles_lo[cls] = 0;
         nq_lo[cls] = 0;
         nprimq_lo[cls] = 0.0;
@@ -619,8 +638,34 @@ int GtfSystem::build_class_tiles(int cls)
     return 0;
 }
 
-int GtfSystem::jk(const double* D, double* J, double* K)
+bool GtfSystem::fits_densities(int nd) const
 {
+    for (int cls = 0; cls < GTF_NCLASSES; cls++)
+        if (ntiles[cls] + ntiles_lo[cls] > 0 && tile_lds[cls] + (size_t)(nd - 1) * tile_lds_work[cls] > max_lds)
+            return false;
+    return true;
+}
+
+int GtfSystem::jk(const double* D, double* J, double* K, int nd, long dstride)
+{
+    if (nd < 1 || (nd > 1 && !fits_densities(nd))) return -1;
+    if (balance) {
+        long tot = 0;
+        for (int cls = 0; cls < GTF_NCLASSES; cls++)
+            for (int lo = 0; lo < 2; lo++) {
+                const int n = nq[cls] == 0 ? 0 : lo ? ntiles_lo[cls] : ntiles[cls];
+                wg_off[2 * cls + lo] = tot;
+                wg_n[2 * cls + lo] = std::min(n, shape[cls].grid);
+                tot += wg_n[2 * cls + lo];
+            }
+        if (tot > wgstat_cap) {
+            SYS_CHECK(hipStreamSynchronize(stream));
+            dfree(d_wgstat);
+            d_wgstat = dalloc<long long>(3 * tot);
+            if (!d_wgstat) return -1;
+            wgstat_cap = tot;
+        }
+    }
     if (gtf_fill_int(stream, d_counters, 2 * GTF_NCLASSES, 0)) return -1;
     if (!cstream[0]) {
         for (int s = 0; s < NSTREAMS; s++) {
@@ -641,7 +686,7 @@ int GtfSystem::jk(const double* D, double* J, double* K)
         if (nq[cls] == 0) continue;
         hipStream_t cs = serial ? stream : cstream[slot++ % NSTREAMS];
         if (timing) SYS_CHECK(hipEventRecord(tev[2 * cls], cs));
-        if (launch_class(cls, cs, D, J, K)) return -1;
+        if (launch_class(cls, cs, D, J, K, nd, dstride)) return -1;
         if (timing) SYS_CHECK(hipEventRecord(tev[2 * cls + 1], cs));
     }
     for (int s = 0; s < NSTREAMS; s++) {
@@ -651,7 +696,7 @@ int GtfSystem::jk(const double* D, double* J, double* K)
     return 0;
 }
 
-int GtfSystem::launch_class(int cls, hipStream_t cs, const double* D, double* J, double* K)
+int GtfSystem::launch_class(int cls, hipStream_t cs, const double* D, double* J, double* K, int nd, long dstride)
 {
     int A, B;
     gtf_class_types(cls, A, B);
@@ -677,6 +722,8 @@ int GtfSystem::launch_class(int cls, hipStream_t cs, const double* D, double* J,
     a.D = D;
     a.J = J;
     a.K = K;
+    a.nd = nd;
+    a.dstride = dstride;
     a.mbase = d_mbase;
     a.mld = d_mld;
     a.shell_mol = d_shell_mol;
@@ -689,24 +736,58 @@ int GtfSystem::launch_class(int cls, hipStream_t cs, const double* D, double* J,
         const bool exch = exchanged(cls);
         const int grid = shape[cls].grid;
         const int bl = shape[cls].boys_lds ? 1 : 0;
+        const size_t lds = tile_lds[cls] + (size_t)(nd - 1) * tile_lds_work[cls];
         if (ntiles[cls] > 0) {
             a.tiles = d_tiles[cls];
             a.ntiles = ntiles[cls];
             a.counter = d_counters + cls;
-            if (launch(exch ? ci->jk_tile_t[bl] : ci->jk_tile[bl], a, std::min(a.ntiles, grid), cs, tile_lds[cls]))
+            a.wg_stat = wg_record(2 * cls, std::min(a.ntiles, grid));
+            if (launch(exch ? ci->jk_tile_t[bl] : ci->jk_tile[bl], a, std::min(a.ntiles, grid), cs, lds))
                 return -1;
         }
         if (ntiles_lo[cls] > 0) {
             a.tiles = d_tiles_lo[cls];
             a.ntiles = ntiles_lo[cls];
             a.counter = d_counters + GTF_NCLASSES + cls;
-            if (launch(exch ? ci->jk_tile_t_lo[bl] : ci->jk_tile_lo[bl], a, std::min(a.ntiles, grid), cs,
-                       tile_lds[cls]))
+            a.wg_stat = wg_record(2 * cls + 1, std::min(a.ntiles, grid));
+            if (launch(exch ? ci->jk_tile_t_lo[bl] : ci->jk_tile_lo[bl], a, std::min(a.ntiles, grid), cs, lds))
                 return -1;
         }
         return 0;
     }
     return launch(ci->jk, a, std::min(a.ntasks, 2048), cs);
+}
+
+int GtfSystem::balance_stats(double* lb, long* steals, long* nwg)
+{
+    *lb = 1.0;
+    *steals = 0;
+    *nwg = 0;
+    if (!balance || !d_wgstat) return 0;
+    SYS_CHECK(hipStreamSynchronize(stream));
+    long tot = 0;
+    for (int k = 0; k < 2 * GTF_NCLASSES; k++) tot = std::max(tot, wg_off[k] + wg_n[k]);
+    std::vector<long long> h(3 * tot);
+    if (tot > 0) SYS_CHECK(hipMemcpy(h.data(), d_wgstat, 3 * tot * sizeof(long long), hipMemcpyDeviceToHost));
+    double smax = 0.0, smean = 0.0;
+    for (int k = 0; k < 2 * GTF_NCLASSES; k++) {
+        if (wg_n[k] == 0) continue;
+        const long long* w = h.data() + 3 * wg_off[k];
+        long long t0 = w[0];
+        for (int g = 1; g < wg_n[k]; g++) t0 = std::min(t0, w[3 * g]);
+        double mx = 0.0, sum = 0.0;
+        for (int g = 0; g < wg_n[k]; g++) {
+            const double d = (double)(w[3 * g + 1] - t0);
+            mx = std::max(mx, d);
+            sum += d;
+            *steals += std::max(0LL, w[3 * g + 2] - 1);
+        }
+        smax += mx;
+        smean += sum / wg_n[k];
+        *nwg += wg_n[k];
+    }
+    if (smean > 0.0) *lb = smax / smean;
+    return 0;
 }
 
 // best-of-reps event time of `run` (which enqueues on this->stream)
@@ -882,6 +963,14 @@ static bool shape_valid(int cls, const int* in)
     for (int f = 4; f < GTF_SHAPE_FIELDS; f++)
         if (in[f] < 0 || in[f] > 1) return false;
     return !(in[4] == 1 && gtf_eri_class(cls)->nsplit_t == 0);
+}
+
+extern "C" int gtf_tile_shapes_check(const int* shapes)
+{
+    if (!shapes) return -3;
+    for (int c = 0; c < GTF_NCLASSES; c++)
+        if (!shape_valid(c, shapes + GTF_SHAPE_FIELDS * c)) return -3;
+    return 0;
 }
 
 void GtfSystem::put_shape(int cls, const int* in)

@@ -97,7 +97,8 @@ struct GtfSystem {
     int ntiles[GTF_NCLASSES] = {0}, ntiles_lo[GTF_NCLASSES] = {0};
     int4* d_tiles[GTF_NCLASSES] = {nullptr};     // FP64 tiles
     int4* d_tiles_lo[GTF_NCLASSES] = {nullptr};  // FP32 tiles
-    size_t tile_lds[GTF_NCLASSES] = {0};
+    size_t tile_lds[GTF_NCLASSES] = {0};       // one density
+    size_t tile_lds_work[GTF_NCLASSES] = {0};  // of that, the work area each further density adds
     size_t max_lds = 65536;
     bool use_tiles = true;
     // Tile shape of each class (the autotuner's search space): at most
@@ -145,6 +146,26 @@ struct GtfSystem {
     hipEvent_t tev[2 * GTF_NCLASSES] = {nullptr};
     double class_ms[GTF_NCLASSES] = {0};
     long class_launches[GTF_NCLASSES] = {0};
+    // Load balance of the tile launches (the PFock statistics): when on,
+    // every tile launch records per workgroup {clock at start, clock at
+    // exit, tiles taken}; launch 2 * cls + (FP32 tiles) owns wg_n entries
+    // from wg_off.
+    bool balance = false;
+    long long* d_wgstat = nullptr;
+    long wgstat_cap = 0;
+    long wg_off[2 * GTF_NCLASSES] = {0};
+    int wg_n[2 * GTF_NCLASSES] = {0};
+    // over the last build's tile launches: lb = sum of the slowest
+    // workgroup's time / sum of the mean workgroup time, steals = tiles
+    // taken after each workgroup's first, nwg = workgroups
+    int balance_stats(double* lb, long* steals, long* nwg);
+    // where launch k records its workgroups: only a launch of the grid the
+    // last build reserved for it (a launch outside a build, or after a shape
+    // change, records nothing)
+    long long* wg_record(int k, int grid) const
+    {
+        return balance && d_wgstat && grid == wg_n[k] ? d_wgstat + 3 * wg_off[k] : nullptr;
+    }
     size_t dev_bytes = 0;
     std::vector<void*> allocs;
 
@@ -168,9 +189,12 @@ struct GtfSystem {
     PairView pair_view() const;
     // fused ERI + J/K digestion over all screened quartets of every class;
     // D, J, K are the packed blocks of all molecules
-    int jk(const double* D, double* J, double* K);
+    // nd densities (and J, K) dstride elements apart, digested from each
+    // integral; nd > 1 needs fits_densities(nd) (LDS work areas per density)
+    int jk(const double* D, double* J, double* K, int nd = 1, long dstride = 0);
+    bool fits_densities(int nd) const;
     // the launches of one class on stream cs (its counters already zero)
-    int launch_class(int cls, hipStream_t cs, const double* D, double* J, double* K);
+    int launch_class(int cls, hipStream_t cs, const double* D, double* J, double* K, int nd = 1, long dstride = 0);
     // Autotuner: per class, in decreasing order of work, a coordinate search
     // over the exchanged kernel, units, unit_pq, tile_pq and grid, each
     // candidate timed with the class alone on the stream (best of reps);

@@ -32,10 +32,13 @@ struct EriArgs {
     PairView pv;
     const int* fstart;
     const double* boys8;  // compact Taylor tables, BOYS8_NM slabs of BOYS8_SLAB
-    // EPI_JK
+    // EPI_JK: nd densities (and their J, K) dstride elements apart, all
+    // digested from each integral (pfock/fock_task.c:46 loops num_dmat)
     const double* D;
     double* J;
     double* K;
+    int nd;
+    long dstride;
     // EPI_DIAG
     unsigned long long* vbits;
     // EPI_STORE
@@ -64,7 +67,22 @@ struct EriArgs {
     // built (0: its quartets are skipped; a molecule whose SCF has ended)
     const int* shell_mol;
     const int* mol_active;
+    // tile kernels (optional): per workgroup {clock at start, clock at exit,
+    // tiles taken}, three entries at 3 * blockIdx.x -- the load balance of a
+    // launch (pfock/pfock.c:1918-2010 reports it across processes)
+    long long* wg_stat;
 };
+
+// one workgroup's record of its part in a tile launch
+__device__ __forceinline__ void record_workgroup(const EriArgs& a, long long t_begin, int taken)
+{
+    if (a.wg_stat && threadIdx.x == 0) {
+        long long* w = a.wg_stat + 3L * blockIdx.x;
+        w[0] = t_begin;
+        w[1] = (long long)wall_clock64();
+        w[2] = taken;
+    }
+}
 
 // the bra pair's molecule takes part in this build
 __device__ __forceinline__ bool bra_active(const EriArgs& a, int bra)
@@ -93,7 +111,8 @@ __device__ __forceinline__ int find_row(const long* __restrict__ off, int nrows,
 template <int LA, int LB, int LC, int LD, int NC, bool STRIP>
 __device__ __forceinline__ void digest(const EriArgs& a, int bra, int ket, int c0,
                                        const double (&acc)[NC][EriShape<LA, LB, LC, LD>::NCD], double* jmn,
-                                       double* strip, int nbf, double* jpq, double* jmn_reg = nullptr);
+                                       double* strip, int nbf, double* jpq, double* jmn_reg = nullptr,
+                                       long doff = 0);
 
 template <int LA, int LB, int LC, int LD, int NC, int EPI>
 __device__ __forceinline__ void eri_unit(const EriArgs& a, long u)
@@ -142,7 +161,9 @@ __device__ __forceinline__ void eri_unit(const EriArgs& a, long u)
         });
         atomicMax(a.vbits + bra, (unsigned long long)__double_as_longlong(mx));
     } else {
-        digest<LA, LB, LC, LD, NC, false>(a, bra, ket, c0, acc, nullptr, nullptr, 0, nullptr);
+        for (int d = 0; d < a.nd; d++)
+            digest<LA, LB, LC, LD, NC, false>(a, bra, ket, c0, acc, nullptr, nullptr, 0, nullptr, nullptr,
+                                              d * a.dstride);
     }
 }
 
@@ -158,7 +179,7 @@ __device__ __forceinline__ void eri_unit(const EriArgs& a, long u)
 template <int LA, int LB, int LC, int LD, int NC, bool STRIP>
 __device__ __forceinline__ void digest(const EriArgs& a, int bra, int ket, int c0,
                                        const double (&acc)[NC][EriShape<LA, LB, LC, LD>::NCD], double* jmn,
-                                       double* strip, int nbf, double* jpq, double* jmn_reg)
+                                       double* strip, int nbf, double* jpq, double* jmn_reg, long doff)
 {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr int NCD = S::NCD, ND = S::ND, NB = S::NB, NA = S::NA;
@@ -170,9 +191,9 @@ __device__ __forceinline__ void digest(const EriArgs& a, int bra, int ket, int c
     const int ld = a.mld[M];
     const long base = a.mbase[M];
     const int oM = a.fstart[M], oN = a.fstart[N], oP = a.fstart[P], oQ = a.fstart[Q];
-    const double* __restrict__ D = a.D + base;
-    double* J = a.J + base;
-    double* K = a.K + base;
+    const double* __restrict__ D = a.D + doff + base;
+    double* J = a.J + doff + base;
+    double* K = a.K + doff + base;
     int mloc[NC], nloc[NC];
     double dmn[NC];
     static_for<0, NC>([&](auto c_) {
@@ -357,11 +378,14 @@ __global__ __launch_bounds__(256, (tile_min_waves<LA, LB, LC, LD>())) void eri_j
     __shared__ int s_task;
     double* work;
     const double* boys = tile_boys<L, BLDS>(a, smem, work);
+    const long long t_begin = (long long)wall_clock64();
+    int taken = 0;
     for (;;) {
         if (threadIdx.x == 0) s_task = atomicAdd(a.counter, 1);
         __syncthreads();
         const int task = s_task;
         if (task >= a.ntiles) break;
+        taken++;
         const int4 t = a.tiles[task];
         const int bra = a.bra_list[t.x];
         if (!bra_active(a, bra)) {
@@ -377,7 +401,9 @@ __global__ __launch_bounds__(256, (tile_min_waves<LA, LB, LC, LD>())) void eri_j
         double* jmn = work + nstrip;
         double* jpq = jmn + NA * NB;
         const int njpq = JPQ_LDS ? (t.z - t.y) * NCD : 0;
-        for (int k = threadIdx.x; k < nstrip + NA * NB + njpq; k += 256) work[k] = 0.0;
+        // one work area (strip, J_MN, J_PQ) per density
+        const int wsz = nstrip + NA * NB + njpq;
+        for (int k = threadIdx.x; k < a.nd * wsz; k += 256) work[k] = 0.0;
         __syncthreads();
         // Unit order.  NSPLIT == 1: the nic chunks of one quartet sit in
         // neighbouring lanes (nic is a power of two, so a group never spans
@@ -430,8 +456,10 @@ __global__ __launch_bounds__(256, (tile_min_waves<LA, LB, LC, LD>())) void eri_j
                     });
                 if (ic != 0) continue;
             }
-            digest<LA, LB, LC, LD, NC, true>(a, bra, ket, c0, acc, jmn, strip, nbf,
-                                             JPQ_LDS ? jpq + q * NCD : nullptr, NSPLIT == 1 ? jreg : nullptr);
+            for (int d = 0; d < a.nd; d++)
+                digest<LA, LB, LC, LD, NC, true>(a, bra, ket, c0, acc, jmn + d * wsz, strip + d * wsz, nbf,
+                                                 JPQ_LDS ? jpq + d * wsz + q * NCD : nullptr,
+                                                 NSPLIT == 1 && a.nd == 1 ? jreg : nullptr, d * a.dstride);
         }
         if constexpr (NSPLIT == 1) {
             static_for<0, NC>([&](auto c_) {
@@ -444,31 +472,39 @@ __global__ __launch_bounds__(256, (tile_min_waves<LA, LB, LC, LD>())) void eri_j
         const int oM = a.fstart[M], oN = a.fstart[N];
         const long base = a.mbase[M];
         const int ld = a.mld[M];
-        for (int k = threadIdx.x; k < nstrip; k += 256) {
-            const double v = strip[k];
-            if (v != 0.0) {
-                const int r = k / nbf, col = k - r * nbf;
-                const int row = r < NA ? oM + r : oN + (r - NA);
-                GTF_ATOMIC_ADD(&a.K[base + (long)row * ld + col], v);
-            }
-        }
-        for (int k = threadIdx.x; k < NA * NB; k += 256) {
-            const double v = jmn[k];
-            if (v != 0.0) GTF_ATOMIC_ADD(&a.J[base + (long)(oM + k / NB) * ld + oN + k % NB], v);
-        }
-        if constexpr (JPQ_LDS) {
-            for (int k = threadIdx.x; k < njpq; k += 256) {
-                const double v = jpq[k];
+        for (int d = 0; d < a.nd; d++) {
+            const double* sd = strip + d * wsz;
+            const double* jd = jmn + d * wsz;
+            double* Kd = a.K + d * a.dstride + base;
+            double* Jd = a.J + d * a.dstride + base;
+            for (int k = threadIdx.x; k < nstrip; k += 256) {
+                const double v = sd[k];
                 if (v != 0.0) {
-                    const int q = k / NCD, cd = k - q * NCD;
-                    const int ket = a.ket_list[t.y + q];
-                    const int oP = a.fstart[a.pv.s1[ket]], oQ = a.fstart[a.pv.s2[ket]];
-                    GTF_ATOMIC_ADD(&a.J[base + (long)(oP + cd / ND) * ld + oQ + cd % ND], v);
+                    const int r = k / nbf, col = k - r * nbf;
+                    const int row = r < NA ? oM + r : oN + (r - NA);
+                    GTF_ATOMIC_ADD(&Kd[(long)row * ld + col], v);
+                }
+            }
+            for (int k = threadIdx.x; k < NA * NB; k += 256) {
+                const double v = jd[k];
+                if (v != 0.0) GTF_ATOMIC_ADD(&Jd[(long)(oM + k / NB) * ld + oN + k % NB], v);
+            }
+            if constexpr (JPQ_LDS) {
+                const double* pd = jpq + d * wsz;
+                for (int k = threadIdx.x; k < njpq; k += 256) {
+                    const double v = pd[k];
+                    if (v != 0.0) {
+                        const int q = k / NCD, cd = k - q * NCD;
+                        const int ket = a.ket_list[t.y + q];
+                        const int oP = a.fstart[a.pv.s1[ket]], oQ = a.fstart[a.pv.s2[ket]];
+                        GTF_ATOMIC_ADD(&Jd[(long)(oP + cd / ND) * ld + oQ + cd % ND], v);
+                    }
                 }
             }
         }
         __syncthreads();
     }
+    record_workgroup(a, t_begin, taken);
 }
 
 // update_F digestion of a quartet evaluated with bra and ket exchanged: the
@@ -480,7 +516,8 @@ __global__ __launch_bounds__(256, (tile_min_waves<LA, LB, LC, LD>())) void eri_j
 template <int LA, int LB, int LC, int LD, int NCT>
 __device__ __forceinline__ void digest_t(const EriArgs& a, int bra, int ket, int cd0,
                                          const double (&acc)[NCT][EriShape<LA, LB, LC, LD>::NAB],
-                                         double* jmn_reg, double* strip, int nbf, double* jpq)
+                                         double* jmn_reg, double* strip, int nbf, double* jpq, double* jmn_lds,
+                                         long doff)
 {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr int NAB = S::NAB, NA = S::NA, NB = S::NB, ND = S::ND;
@@ -492,18 +529,20 @@ __device__ __forceinline__ void digest_t(const EriArgs& a, int bra, int ket, int
     const int ld = a.mld[M];
     const long base = a.mbase[M];
     const int oM = a.fstart[M], oN = a.fstart[N], oP = a.fstart[P], oQ = a.fstart[Q];
-    const double* __restrict__ D = a.D + base;
+    const double* __restrict__ D = a.D + doff + base;
     static_for<0, NCT>([&](auto k_) {
         constexpr int k = decltype(k_)::value;
         const int cd = cd0 + k;
         const int p = cd / ND, q = cd - (cd / ND) * ND;
         const double dpq = D[(long)(oP + p) * ld + oQ + q];
-        // J_MN += cJ1 D_pq I  and  J_PQ += cJ2 sum_mn D_mn I
+        // J_MN += cJ1 D_pq I (registers, or LDS with several densities) and
+        // J_PQ += cJ2 sum_mn D_mn I
         double jpq_s = 0.0;
         static_for<0, NAB>([&](auto ab_) {
             constexpr int ab = decltype(ab_)::value;
             constexpr int m = ab / NB, n = ab % NB;
-            jmn_reg[ab] += cJ1 * dpq * acc[k][ab];
+            if (jmn_lds) atomicAdd(&jmn_lds[ab], cJ1 * dpq * acc[k][ab]);
+            else jmn_reg[ab] += cJ1 * dpq * acc[k][ab];
             jpq_s += D[(long)(oM + m) * ld + oN + n] * acc[k][ab];
         });
         if (f3) atomicAdd(&jpq[cd], cJ2 * jpq_s);
@@ -555,11 +594,14 @@ __global__ __launch_bounds__(256) void eri_jk_tile_t_kernel(EriArgs a)
     __shared__ int s_task;
     double* work;
     const double* boys = tile_boys<L, BLDS>(a, smem, work);
+    const long long t_begin = (long long)wall_clock64();
+    int taken = 0;
     for (;;) {
         if (threadIdx.x == 0) s_task = atomicAdd(a.counter, 1);
         __syncthreads();
         const int task = s_task;
         if (task >= a.ntiles) break;
+        taken++;
         const int4 t = a.tiles[task];
         const int bra = a.bra_list[t.x];
         if (!bra_active(a, bra)) {
@@ -573,7 +615,8 @@ __global__ __launch_bounds__(256) void eri_jk_tile_t_kernel(EriArgs a)
         double* jpq = jmn + NA * NB;
         const int nkt = t.z - t.y;
         const int njpq = nkt * NCD;
-        for (int k = threadIdx.x; k < nstrip + NA * NB + njpq; k += 256) work[k] = 0.0;
+        const int wsz = nstrip + NA * NB + njpq;  // work area per density
+        for (int k = threadIdx.x; k < a.nd * wsz; k += 256) work[k] = 0.0;
         __syncthreads();
         double jreg[NAB];
         static_for<0, NAB>([&](auto c_) { jreg[decltype(c_)::value] = 0.0; });
@@ -598,40 +641,52 @@ __global__ __launch_bounds__(256) void eri_jk_tile_t_kernel(EriArgs a)
             } else {
                 eri_block<LC, LD, LA, LB, NCT, double>(a.pv, ket, bra, cd0, boys, acc);
             }
-            digest_t<LA, LB, LC, LD, NCT>(a, bra, ket, cd0, acc, jreg, strip, nbf, jpq + q * NCD);
+            for (int d = 0; d < a.nd; d++)
+                digest_t<LA, LB, LC, LD, NCT>(a, bra, ket, cd0, acc, jreg, strip + d * wsz, nbf,
+                                              jpq + d * wsz + q * NCD, a.nd == 1 ? nullptr : jmn + d * wsz,
+                                              d * a.dstride);
         }
-        static_for<0, NAB>([&](auto c_) {
-            constexpr int c = decltype(c_)::value;
-            if (jreg[c] != 0.0) atomicAdd(&jmn[c], jreg[c]);
-        });
+        if (a.nd == 1)
+            static_for<0, NAB>([&](auto c_) {
+                constexpr int c = decltype(c_)::value;
+                if (jreg[c] != 0.0) atomicAdd(&jmn[c], jreg[c]);
+            });
         __syncthreads();
         const int M = a.pv.s1[bra], N = a.pv.s2[bra];
         const int oM = a.fstart[M], oN = a.fstart[N];
         const long base = a.mbase[M];
         const int ld = a.mld[M];
-        for (int k = threadIdx.x; k < nstrip; k += 256) {
-            const double v = strip[k];
-            if (v != 0.0) {
-                const int r = k / nbf, col = k - r * nbf;
-                const int row = r < NA ? oM + r : oN + (r - NA);
-                GTF_ATOMIC_ADD(&a.K[base + (long)row * ld + col], v);
+        for (int d = 0; d < a.nd; d++) {
+            const double* sd = strip + d * wsz;
+            const double* jd = jmn + d * wsz;
+            const double* pd = jpq + d * wsz;
+            double* Kd = a.K + d * a.dstride + base;
+            double* Jd = a.J + d * a.dstride + base;
+            for (int k = threadIdx.x; k < nstrip; k += 256) {
+                const double v = sd[k];
+                if (v != 0.0) {
+                    const int r = k / nbf, col = k - r * nbf;
+                    const int row = r < NA ? oM + r : oN + (r - NA);
+                    GTF_ATOMIC_ADD(&Kd[(long)row * ld + col], v);
+                }
             }
-        }
-        for (int k = threadIdx.x; k < NA * NB; k += 256) {
-            const double v = jmn[k];
-            if (v != 0.0) GTF_ATOMIC_ADD(&a.J[base + (long)(oM + k / NB) * ld + oN + k % NB], v);
-        }
-        for (int k = threadIdx.x; k < njpq; k += 256) {
-            const double v = jpq[k];
-            if (v != 0.0) {
-                const int q = k / NCD, cd = k - q * NCD;
-                const int ket = a.ket_list[t.y + q];
-                const int oP = a.fstart[a.pv.s1[ket]], oQ = a.fstart[a.pv.s2[ket]];
-                GTF_ATOMIC_ADD(&a.J[base + (long)(oP + cd / ND) * ld + oQ + cd % ND], v);
+            for (int k = threadIdx.x; k < NA * NB; k += 256) {
+                const double v = jd[k];
+                if (v != 0.0) GTF_ATOMIC_ADD(&Jd[(long)(oM + k / NB) * ld + oN + k % NB], v);
+            }
+            for (int k = threadIdx.x; k < njpq; k += 256) {
+                const double v = pd[k];
+                if (v != 0.0) {
+                    const int q = k / NCD, cd = k - q * NCD;
+                    const int ket = a.ket_list[t.y + q];
+                    const int oP = a.fstart[a.pv.s1[ket]], oQ = a.fstart[a.pv.s2[ket]];
+                    GTF_ATOMIC_ADD(&Jd[(long)(oP + cd / ND) * ld + oQ + cd % ND], v);
+                }
             }
         }
         __syncthreads();
     }
+    record_workgroup(a, t_begin, taken);
 }
 
 // launch helpers (eri_table.cpp)

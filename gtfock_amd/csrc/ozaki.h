@@ -157,19 +157,27 @@ __device__ __forceinline__ void oz_gemm_tile(int ld, const __bf16* __restrict__ 
     const int lr = tid >> 2, lk = (tid & 3) * 8;
     const __bf16* ga = QA + (long)(row0 + lr) * ld + lk;
     const __bf16* gb = QB + (long)(col0 + lr) * ld + lk;
-    int kc = 0;
-    for (int k0 = 0; k0 < ld; k0 += OZ_KB) {
-        uint4 va[NS], vb[NS];
+    // the next K block's slices are loaded into registers while the current
+    // one is multiplied out of LDS
+    uint4 va[NS], vb[NS];
+    auto load = [&](int k0) {
         for (int i = 0; i < NS; i++) {
             va[i] = *reinterpret_cast<const uint4*>(ga + i * sl + k0);
             vb[i] = *reinterpret_cast<const uint4*>(gb + i * sl + k0);
         }
-        __syncthreads();
+    };
+    auto store = [&]() {
         for (int i = 0; i < NS; i++) {
             *reinterpret_cast<uint4*>(&As[i][lr][lk]) = va[i];
             *reinterpret_cast<uint4*>(&Bs[i][lr][lk]) = vb[i];
         }
-        __syncthreads();
+    };
+    load(0);
+    store();
+    __syncthreads();
+    int kc = 0;
+    for (int k0 = 0; k0 < ld; k0 += OZ_KB) {
+        if (k0 + OZ_KB < ld) load(k0 + OZ_KB);
         for (int kk = 0; kk < OZ_KB; kk += 16) {
             oz_bf16x8_t a[NS], b[NS];
             for (int i = 0; i < NS; i++) {
@@ -183,6 +191,11 @@ __device__ __forceinline__ void oz_gemm_tile(int ld, const __bf16* __restrict__ 
                     acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[d - i], acc[d], 0, 0, 0);
                 });
             });
+        }
+        __syncthreads();  // every wave is done with this block
+        if (k0 + OZ_KB < ld) {
+            store();
+            __syncthreads();
         }
         kc += OZ_KB;
         if (kc == OZ_KCHUNK || k0 + OZ_KB >= ld) {

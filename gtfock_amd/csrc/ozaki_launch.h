@@ -5,6 +5,12 @@
 // slice counts the product kernels are built for
 #define OZ_SLICES_MIN 3
 #define OZ_SLICES_MAX 6
+// purification defaults: slices per operand, and the DIIS residual
+// ||X (F D S - S D F) X||_F below which an SCF's purifications run FP64
+// throughout (the split products' rounding would otherwise set the floor
+// of the SCF's energy changes)
+#define OZ_SLICES_DEFAULT 5
+#define OZ_SPLIT_RESID 1e-3
 // purification product modes (per molecule in a batch): FP64, FP32 MFMA,
 // split BF16
 #define GTF_LOW_FP64 0

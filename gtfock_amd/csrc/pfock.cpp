@@ -32,6 +32,7 @@ struct PFock {
     int max_numdmat = 1, num_dmat = 1, nosymm = 0;
     double tolscr = 0, tolscr2 = 0;
     int committed = 0;
+    int fused_nd = 0;  // densities the last build digested from each integral
     std::vector<double*> d_D, d_J, d_K;
     double* d_F = nullptr;
     double *d_S = nullptr, *d_H = nullptr, *d_X = nullptr;
@@ -123,6 +124,7 @@ PFockStatus_t PFock_create(BasisSet_t basis, int nprow, int npcol, int ntasks, d
     (void)hipEventCreate(&pf->ev0);
     (void)hipEventCreate(&pf->ev1);
     double t0 = wtime();
+    pf->sys.balance = true;  // workgroup records for PFock_getStatistics
     int rc = pf->sys.init(basis, pf->stream);
     if (rc == 0) rc = pf->sys.screen(tolscr);
     if (rc != 0) {
@@ -134,16 +136,20 @@ PFockStatus_t PFock_create(BasisSet_t basis, int nprow, int npcol, int ntasks, d
     pf->sys.csr(pf->shellptr, pf->shellid, pf->shellrid, pf->shellvalue);
     pf->nnz = (int)pf->shellid.size();
     const long nn = (long)pf->nbf * pf->nbf;
+    // the densities (and their J, K) one after another, nn apart, so one
+    // build digests every committed density from each integral
+    double *dd = pf->sys.dalloc<double>(nn * max_numdmat), *jj = pf->sys.dalloc<double>(nn * max_numdmat),
+           *kk = pf->sys.dalloc<double>(nn * max_numdmat);
+    if (!dd || !jj || !kk) { PFock_destroy(pf); return PFOCK_STATUS_ALLOC_FAILED; }
+    if (gtf_fill(pf->stream, dd, nn * max_numdmat, 0.0) || gtf_fill(pf->stream, jj, nn * max_numdmat, 0.0) ||
+        gtf_fill(pf->stream, kk, nn * max_numdmat, 0.0)) {
+        PFock_destroy(pf);
+        return PFOCK_STATUS_EXECUTION_FAILED;
+    }
     for (int i = 0; i < max_numdmat; i++) {
-        double *d = pf->sys.dalloc<double>(nn), *j = pf->sys.dalloc<double>(nn), *k = pf->sys.dalloc<double>(nn);
-        if (!d || !j || !k) { PFock_destroy(pf); return PFOCK_STATUS_ALLOC_FAILED; }
-        pf->d_D.push_back(d);
-        pf->d_J.push_back(j);
-        pf->d_K.push_back(k);
-        if (gtf_fill(pf->stream, d, nn, 0.0) || gtf_fill(pf->stream, j, nn, 0.0) || gtf_fill(pf->stream, k, nn, 0.0)) {
-            PFock_destroy(pf);
-            return PFOCK_STATUS_EXECUTION_FAILED;
-        }
+        pf->d_D.push_back(dd + i * nn);
+        pf->d_J.push_back(jj + i * nn);
+        pf->d_K.push_back(kk + i * nn);
     }
     pf->d_F = pf->sys.dalloc<double>(nn);
     if (!pf->d_F) { PFock_destroy(pf); return PFOCK_STATUS_ALLOC_FAILED; }
@@ -333,14 +339,25 @@ PFockStatus_t PFock_computeFock(BasisSet_t basis, PFock_t pf)
     const double t0 = wtime();
     const long nn = (long)pf->nbf * pf->nbf;
     PF_HIP(hipEventRecord(pf->ev0, pf->stream), "hipEventRecord");
-    for (int i = 0; i < pf->num_dmat; i++) {
-        if (gtf_fill(pf->stream, pf->d_J[i], nn, 0.0) || gtf_fill(pf->stream, pf->d_K[i], nn, 0.0))
-            return PFOCK_STATUS_EXECUTION_FAILED;
-        if (pf->sys.jk(pf->d_D[i], pf->d_J[i], pf->d_K[i])) return PFOCK_STATUS_EXECUTION_FAILED;
+    const int nd = pf->num_dmat;
+    if (gtf_fill(pf->stream, pf->d_J[0], nn * nd, 0.0) || gtf_fill(pf->stream, pf->d_K[0], nn * nd, 0.0))
+        return PFOCK_STATUS_EXECUTION_FAILED;
+    // every density from each integral (update_F's num_dmat loop,
+    // pfock/fock_task.c:46) when the per-density LDS work areas fit, else
+    // one build per density
+    if (nd == 1 || pf->sys.fits_densities(nd)) {
+        if (pf->sys.jk(pf->d_D[0], pf->d_J[0], pf->d_K[0], nd, nn)) return PFOCK_STATUS_EXECUTION_FAILED;
         if (pf->sys.collect_times()) return PFOCK_STATUS_EXECUTION_FAILED;
-        if (pf->sys.symmetrize(pf->d_J[i]) || pf->sys.symmetrize(pf->d_K[i]))
-            return PFOCK_STATUS_EXECUTION_FAILED;
+        pf->fused_nd = nd;
+    } else {
+        pf->fused_nd = 1;
+        for (int i = 0; i < nd; i++) {
+            if (pf->sys.jk(pf->d_D[i], pf->d_J[i], pf->d_K[i])) return PFOCK_STATUS_EXECUTION_FAILED;
+            if (pf->sys.collect_times()) return PFOCK_STATUS_EXECUTION_FAILED;
+        }
     }
+    for (int i = 0; i < nd; i++)
+        if (pf->sys.symmetrize(pf->d_J[i]) || pf->sys.symmetrize(pf->d_K[i])) return PFOCK_STATUS_EXECUTION_FAILED;
     PF_HIP(hipEventRecord(pf->ev1, pf->stream), "hipEventRecord");
     PF_HIP(hipEventSynchronize(pf->ev1), "hipEventSynchronize");
     float ms = 0.f;
@@ -496,13 +513,34 @@ PFockStatus_t PFock_getStatistics(PFock_t pf)
            "      uitl = %.4g (lb = %.3g)\n"
            "      nsq = %.4g (screening = %.3g)\n",
            pf->usq, 1.0, pf->uitl, 1.0, pf->tsq, pf->tsq > 0 ? pf->usq / pf->tsq : 0.0);
-    printf("      load blance = %.3lf\n", 1.0);
-    printf("      steals = 0 (average = 0)\n");
+    // One process: the balance is that of the persistent workgroups of the
+    // last build's tile launches (slowest workgroup over the mean, summed
+    // over the launches); a steal is a tile a workgroup takes from the
+    // shared class counter after its first.
+    double lb = 1.0;
+    long steals = 0, nwg = 0;
+    if (pf->sys.balance_stats(&lb, &steals, &nwg)) return PFOCK_STATUS_EXECUTION_FAILED;
+    printf("      load blance = %.3lf\n", lb);
+    printf("      steals = %.3g (average = %.3g)\n", (double)steals, nwg > 0 ? (double)steals / nwg : 0.0);
     fflush(stdout);
     return PFOCK_STATUS_SUCCESS;
 }
 
 // ---- extra entry points (include/gtfock_amd.h) --------------------------------
+
+int gtf_pfock_fused_densities(PFock_t pf) { return pf ? pf->fused_nd : -1; }
+
+int gtf_pfock_balance(PFock_t pf, double* out)
+{
+    if (!pf || !out) return -1;
+    double lb = 1.0;
+    long steals = 0, nwg = 0;
+    if (pf->sys.balance_stats(&lb, &steals, &nwg)) return -1;
+    out[0] = lb;
+    out[1] = (double)steals;
+    out[2] = (double)nwg;
+    return 0;
+}
 
 int gtf_pfock_stats(PFock_t pf, double* out)
 {

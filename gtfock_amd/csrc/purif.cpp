@@ -16,6 +16,7 @@
 
 #include "../../include/purif.h"
 #include "engine.h"
+#include "batch_linalg.h"
 #include "linalg.h"
 #include "ozaki_launch.h"
 #include "screen.h"
@@ -33,11 +34,27 @@ struct purif_dev {
     // operands, mode 2 = split-BF16 MFMA with ns exact slices (ozaki.h)
     int mode = 0;
     double low_until = 1e-3;
-    int ns = 5;
+    int ns = OZ_SLICES_DEFAULT;
     int low_iters = 0;
+    // mode 2 runs the split products only while the last DIIS residual
+    // ||X (F D S - S D F) X||_F exceeds split_resid (before the SCF's first
+    // residual: split); closer to convergence the purification is FP64
+    double split_resid = OZ_SPLIT_RESID;
+    double last_resid = -1.0;
     void* oz_mem = nullptr;
     OzScratch oz;
+    // the purification loop runs on the device through the batch kernels
+    // with a one-molecule layout: the trace-correcting coefficient, the
+    // stop test and the precision switch are device state, so the host
+    // checks for convergence only every PURIF_CHECK iterations
+    long *d_off = nullptr, *d_poff = nullptr;
+    int *d_n = nullptr, *d_pld = nullptr, *d_nocc = nullptr, *d_iters = nullptr, *d_low = nullptr,
+        *d_lowit = nullptr;
+    double *d_part = nullptr, *d_pcoef = nullptr, *d_stats = nullptr;
+    BatchLayout layout() const { return BatchLayout{1, n, ld, d_off, d_n, d_poff, d_pld, d_part, d_pcoef}; }
 };
+
+#define PURIF_CHECK 4
 
 static double wtime()
 {
@@ -76,21 +93,6 @@ static int load_xs(purif_t* p)
     if (d->xs_loaded) return 0;
     if (h2d(d, d->X, p->X_block, p->ldx) || h2d(d, d->S, p->S_block, p->ldx)) return -1;
     d->xs_loaded = true;
-    return 0;
-}
-
-// D2 = D * D and D3 = D * D2 in split BF16 (ozaki.h), timed like gemm_pair
-static int split_square_cube(purif_dev* d, double* acc)
-{
-    PD_CHECK(hipEventRecord(d->e0, d->stream));
-    OzOperands o;
-    o.maxld = d->ld;
-    if (gtf_oz_square_cube(d->stream, o, d->oz, d->D, d->D2, d->D3)) return -1;
-    PD_CHECK(hipEventRecord(d->e1, d->stream));
-    PD_CHECK(hipEventSynchronize(d->e1));
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, d->e0, d->e1);
-    if (acc) *acc += 1e-3 * ms;
     return 0;
 }
 
@@ -158,7 +160,24 @@ purif_t* create_purif(BasisSet_t basis, int nprow_purif, int npcol_purif, int np
     ok = ok && hipMalloc((void**)&d->scal, sizeof(double) * 64) == hipSuccess;
     ok = ok && hipMalloc((void**)&d->coef, sizeof(double) * 64) == hipSuccess;
     ok = ok && hipMalloc((void**)&d->partial, gtf_partial_bytes()) == hipSuccess;
+    ok = ok && hipMalloc((void**)&d->d_off, sizeof(long) * 2) == hipSuccess;
+    ok = ok && hipMalloc((void**)&d->d_poff, sizeof(long) * 2) == hipSuccess;
+    int** ints[] = {&d->d_n, &d->d_pld, &d->d_nocc, &d->d_iters, &d->d_low, &d->d_lowit};
+    for (int** q : ints) ok = ok && hipMalloc((void**)q, sizeof(int) * 2) == hipSuccess;
+    ok = ok && hipMalloc((void**)&d->d_part, sizeof(double) * BPART * BMAX_DOTS) == hipSuccess;
+    ok = ok && hipMalloc((void**)&d->d_pcoef, sizeof(double) * 2) == hipSuccess;
+    ok = ok && hipMalloc((void**)&d->d_stats, sizeof(double) * 3) == hipSuccess;
     if (!ok) { destroy_purif(p); return nullptr; }
+    {
+        const long zero = 0;
+        const int nn = n, ldv = d->ld, no = p->nobtls;
+        ok = hipMemcpy(d->d_off, &zero, sizeof zero, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(d->d_poff, &zero, sizeof zero, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(d->d_n, &nn, sizeof nn, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(d->d_pld, &ldv, sizeof ldv, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(d->d_nocc, &no, sizeof no, hipMemcpyHostToDevice) == hipSuccess;
+        if (!ok) { destroy_purif(p); return nullptr; }
+    }
     for (double** b : bufs) ok = ok && gtf_fill(d->stream, *b, (long)m, 0.0) == 0;
     ok = ok && gtf_fill(d->stream, d->evecs, (long)m * MAX_DIIS, 0.0) == 0;
     ok = ok && gtf_fill(d->stream, d->Fvecs, (long)m * MAX_DIIS, 0.0) == 0;
@@ -173,9 +192,10 @@ void destroy_purif(purif_t* p)
     purif_dev* d = p->dev;
     if (d) {
         if (d->stream) (void)hipStreamSynchronize(d->stream);
-        double* bufs[] = {d->F, d->D, d->D2, d->D3, d->X, d->S, d->W, d->W2, d->evecs, d->Fvecs,
-                          d->scal, d->coef, d->partial, (double*)d->oz_mem};
-        for (double* b : bufs)
+        void* bufs[] = {d->F, d->D, d->D2, d->D3, d->X, d->S, d->W, d->W2, d->evecs, d->Fvecs,
+                        d->scal, d->coef, d->partial, d->oz_mem, d->d_off, d->d_poff, d->d_n, d->d_pld,
+                        d->d_nocc, d->d_iters, d->d_low, d->d_lowit, d->d_part, d->d_pcoef, d->d_stats};
+        for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (d->e0) (void)hipEventDestroy(d->e0);
         if (d->e1) (void)hipEventDestroy(d->e1);
@@ -190,44 +210,59 @@ void destroy_purif(purif_t* p)
     free(p);
 }
 
+// Palser-Manolopoulos purification (pscf/purif.c:257-488) of F (orthogonal
+// basis), then D_AO = X D X.  Each iteration: D2 = D D, D3 = D D2 (FP64
+// MFMA, or in the reduced precision until ||D - D^2||_F < low_until), the
+// statistics ||D - D2||_F, tr(D2 - D3), tr(D - D2) and the update with
+// c = tr(D2 - D3) / tr(D - D2), all on the device; the molecule is marked
+// done at the iteration whose error is below 1e-11 and later launches skip
+// it, so the host only checks every PURIF_CHECK iterations.
 static int purify(purif_t* p, double* F_block, double* D_block, int* iters)
 {
     purif_dev* d = p->dev;
-    const int n = d->n, ld = d->ld, nocc = p->nobtls;
-    const long m = (long)ld * ld;
     if (load_xs(p) || h2d(d, d->F, F_block, p->ldx)) return -1;
-    double h[3];
-    if (gtf_gershgorin(d->stream, d->F, n, ld, d->scal)) return -1;
-    PD_CHECK(hipMemcpyAsync(h, d->scal, sizeof h, hipMemcpyDeviceToHost, d->stream));
-    PD_CHECK(hipStreamSynchronize(d->stream));
-    const double hmin = h[0], hmax = h[1], mu = h[2] / n;
-    const double lambda = std::min((double)nocc / (hmax - mu), (double)(n - nocc) / (mu - hmin));
-    if (gtf_purif_init(d->stream, d->F, d->D, n, ld, -lambda / n, lambda * mu / n + (double)nocc / n)) return -1;
-    int it;
-    bool low = d->mode != 0;
-    d->low_iters = 0;
+    const BatchLayout L = d->layout();
+    if (d->mode == GTF_LOW_SPLIT && !d->oz_mem) return -1;
+    const int mode =
+        d->mode == GTF_LOW_SPLIT && d->last_resid >= 0.0 && d->last_resid < d->split_resid ? GTF_LOW_FP64 : d->mode;
+    PD_CHECK(hipEventRecord(d->e0, d->stream));
+    if (gtf_bpurif_init(d->stream, L, d->d_nocc, nullptr, d->F, d->D)) return -1;
+    if (gtf_fill_int(d->stream, d->d_iters, 1, -1) || gtf_fill_int(d->stream, d->d_low, 1, mode) ||
+        gtf_fill_int(d->stream, d->d_lowit, 1, 0))
+        return -1;
+    OzOperands o;
+    o.maxld = d->ld;
+    o.done = d->d_iters;
+    o.sel = d->d_low;
+    int it_done = -1, it = 0;
     for (it = 0; it < 200; it++) {
-        if (low && d->mode == 2) {
-            if (split_square_cube(d, &p->timedgemm)) return -1;
-        } else if (gemm_pair(d, d->D, d->D, d->D2, 0, d->D, d->D2, d->D3, 0, &p->timedgemm, low ? 1 : 0)) {
+        if (gtf_bgemm(d->stream, L, nullptr, d->d_iters, d->D, d->D, d->D2, 0, d->d_low) ||
+            gtf_bgemm(d->stream, L, nullptr, d->d_iters, d->D, d->D2, d->D3, 0, d->d_low))
             return -1;
+        if (mode == GTF_LOW_SPLIT && gtf_oz_square_cube(d->stream, o, d->oz, d->D, d->D2, d->D3)) return -1;
+        if (gtf_bpurif_stats(d->stream, L, nullptr, d->D, d->D2, d->D3, d->d_iters, d->d_stats, it, 1e-11, d->d_low,
+                             d->low_until, d->d_lowit))
+            return -1;
+        if (gtf_bpurif_update(d->stream, L, nullptr, d->d_iters, d->d_stats, d->D, d->D2, d->D3, it)) return -1;
+        if (it % PURIF_CHECK == PURIF_CHECK - 1) {
+            PD_CHECK(hipMemcpyAsync(&it_done, d->d_iters, sizeof(int), hipMemcpyDeviceToHost, d->stream));
+            PD_CHECK(hipStreamSynchronize(d->stream));
+            if (it_done >= 0) break;
         }
-        d->low_iters += low ? 1 : 0;
-        const double t0 = wtime();
-        if (gtf_purif_stats(d->stream, d->D, d->D2, d->D3, n, ld, d->partial, d->scal)) return -1;
-        double s[3];
-        PD_CHECK(hipMemcpyAsync(s, d->scal, sizeof s, hipMemcpyDeviceToHost, d->stream));
-        PD_CHECK(hipStreamSynchronize(d->stream));
-        const double err = sqrt(s[0]), c = s[1] / s[2];
-        if (gtf_purif_update(d->stream, d->D, d->D2, d->D3, m, c)) return -1;
-        p->timetr += wtime() - t0;
-        if (low && err < d->low_until) low = false;
-        if (err < 1e-11) break;
     }
+    int lowit = 0;
+    PD_CHECK(hipMemcpyAsync(&it_done, d->d_iters, sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    PD_CHECK(hipMemcpyAsync(&lowit, d->d_lowit, sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    PD_CHECK(hipEventRecord(d->e1, d->stream));
+    PD_CHECK(hipEventSynchronize(d->e1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, d->e0, d->e1);
+    p->timedgemm += 1e-3 * ms;
+    d->low_iters = mode != 0 ? (lowit > 0 ? lowit : (it_done >= 0 ? it_done + 1 : 200)) : 0;
     // D = X D X (pscf/purif.c:469-476)
     if (gemm_pair(d, d->X, d->D, d->W, 1, d->W, d->X, d->D, 1, &p->timedgemm)) return -1;
     if (d2h(d, D_block, d->D, p->ldx)) return -1;
-    *iters = it;
+    *iters = it_done >= 0 ? it_done : 200;
     return 0;
 }
 
@@ -289,6 +324,7 @@ static int diis(purif_t* p, double* D_block, double* F_block, int iter)
     const int LB = MAX_DIIS + 1;
     if (load_xs(p) || h2d(d, d->F, F_block, p->ldx) || h2d(d, d->D, D_block, p->ldx)) return -1;
     double* curF = d->F;
+    if (iter == 0) d->last_resid = -1.0;  // a new SCF
     if (iter > 1) {
         const int cur = p->len_diis < MAX_DIIS ? p->len_diis++ : p->bmax_id;
         if (cur < 0 || cur >= MAX_DIIS) {
@@ -309,6 +345,7 @@ static int diis(purif_t* p, double* D_block, double* F_block, int iter)
             p->b_mat[cur * LB + i] = dots[i];
             p->b_mat[i * LB + cur] = dots[i];
         }
+        d->last_resid = std::sqrt(std::max(dots[cur], 0.0));
         p->bmax = -DBL_MAX;
         for (int i = 0; i < p->len_diis; i++)
             if (p->bmax < p->b_mat[i * LB + i]) {
@@ -344,6 +381,13 @@ int gtf_purif_set_precision(purif_t* p, int mode, double until)
     }
     d->mode = mode;
     d->low_until = until;
+    return 0;
+}
+
+int gtf_purif_set_split_residual(purif_t* p, double resid)
+{
+    if (!p || !p->dev || !(resid >= 0.0)) return -1;
+    p->dev->split_resid = resid;
     return 0;
 }
 

@@ -21,3 +21,26 @@ int gtf_add(hipStream_t st, const double* A, const double* B, double* C, long n)
 int gtf_dgemm(hipStream_t st, int n, int ld, const double* A, const double* B, double* C, int transB);
 // the same with FP32 MFMA on operands rounded to FP32 when fp32 != 0
 int gtf_gemm(hipStream_t st, int n, int ld, const double* A, const double* B, double* C, int transB, int fp32);
+
+// sigsort.hip: significant pairs of one type compacted and ordered on the
+// device.  Scratch of n kept pairs (n from gtf_sig_count's total):
+struct SigWork {
+    int *ids_a, *ids_b, *pos_a, *pos_b, *rank;
+    unsigned long long *k64_a, *k64_b;
+    unsigned *k32_a, *k32_b;
+    void* temp;
+    size_t temp_bytes;
+};
+// outputs: the segment lists in value order (s_id, sorted, sortrank,
+// sortnprim), seg_start / seg_len per (molecule, bucket) (zeroed by the
+// caller), and the ket-list order (sig, sigv, rank, sigmol)
+struct SigOut {
+    int *s_id, *sortrank, *sortnprim, *seg_start, *seg_len, *sig, *rank, *sigmol;
+    double *sorted, *sigv;
+};
+int gtf_sig_count(hipStream_t st, int npairs, const int* type, int t, const unsigned long long* vbits, const int* pmol,
+                  const double* eta, int* block_count, int* block_off, int* total);
+int gtf_sig_order(hipStream_t st, int npairs, const int* type, int t, const unsigned long long* vbits,
+                  const int* pmol, const double* eta, const int* nprim, int nbucket, const int* block_off, int n,
+                  SigWork& w, SigOut& o);
+size_t gtf_sig_temp_bytes(int n);

@@ -13,84 +13,110 @@
 
 // Primitive pairs whose largest scaled coefficient |E| is below prim_eps are
 // dropped (their quartets contribute < 1e-15 to any Fock element); the
-// survivors are packed at the front of the pair's slots and nprim_out
-// receives their count.
-__global__ void build_pairs_kernel(BasisView b, int npairs, const int* __restrict__ s1,
-                                   const int* __restrict__ s2, const int* __restrict__ poff,
-                                   const long* __restrict__ eoff, double4* __restrict__ prim,
-                                   double* __restrict__ E, const long* __restrict__ zoff, double* __restrict__ Ez,
-                                   int* __restrict__ nprim_out, double prim_eps)
+// survivors are packed at the front of the pair's slots, in primitive order,
+// and nprim_out receives their count.
+//
+// One wavefront per shell pair, one lane per primitive pair (chunks of 64):
+// a first pass computes each primitive pair's largest |E|, a ballot gives
+// every survivor its packed slot, and a second pass writes its coefficients
+// there.
+
+// Hermite coefficients of primitive pair (i, j) of shells A, B: returns the
+// largest scaled |E|; with out != null also writes (p, P), the full and the
+// compact coefficient blocks (out, outz, prim_out)
+__device__ double pair_prim(const BasisView& b, int A, int B, int i, int j, double* out, double* outz,
+                            double4* prim_out)
 {
-    const int pid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pid >= npairs) return;
-    const int A = s1[pid], B = s2[pid];
     const int la = b.L[A], lb = b.L[B], lab = la + lb;
-    const int na = nsph(la), nb = nsph(lb), H = nherm(lab), esz = na * nb * H;
+    const int na = nsph(la), nb = nsph(lb), H = nherm(lab);
     const double* CA = b.center + 3 * A;
     const double* CB = b.center + 3 * B;
     double AB2 = 0.0;
     for (int d = 0; d < 3; d++) AB2 += (CA[d] - CB[d]) * (CA[d] - CB[d]);
-    int k = 0;
-    for (int i = 0; i < b.nprim[A]; i++)
-        for (int j = 0; j < b.nprim[B]; j++) {
-            const double a = b.alpha[b.poff[A] + i], bb = b.alpha[b.poff[B] + j];
-            const double p = a + bb, mu = a * bb / p;
-            double P[3];
-            double E1[3][GTF_LMAX + 1][GTF_LMAX + 1][2 * GTF_LMAX + 2];
-            for (int d = 0; d < 3; d++) {
-                P[d] = (a * CA[d] + bb * CB[d]) / p;
-                const double XPA = P[d] - CA[d], XPB = P[d] - CB[d];
-                for (int x = 0; x <= GTF_LMAX; x++)
-                    for (int y = 0; y <= GTF_LMAX; y++)
-                        for (int t = 0; t < 2 * GTF_LMAX + 2; t++) E1[d][x][y][t] = 0.0;
-                E1[d][0][0][0] = 1.0;
-                for (int x = 0; x < la; x++)
-                    for (int t = 0; t <= x + 1; t++) {
-                        double v = XPA * E1[d][x][0][t] + (t + 1) * E1[d][x][0][t + 1];
-                        if (t > 0) v += E1[d][x][0][t - 1] / (2.0 * p);
-                        E1[d][x + 1][0][t] = v;
-                    }
-                for (int y = 0; y < lb; y++)
-                    for (int x = 0; x <= la; x++)
-                        for (int t = 0; t <= x + y + 1; t++) {
-                            double v = XPB * E1[d][x][y][t] + (t + 1) * E1[d][x][y][t + 1];
-                            if (t > 0) v += E1[d][x][y][t - 1] / (2.0 * p);
-                            E1[d][x][y + 1][t] = v;
-                        }
+    const double a = b.alpha[b.poff[A] + i], bb = b.alpha[b.poff[B] + j];
+    const double p = a + bb, mu = a * bb / p;
+    double P[3];
+    double E1[3][GTF_LMAX + 1][GTF_LMAX + 1][2 * GTF_LMAX + 2];
+    for (int d = 0; d < 3; d++) {
+        P[d] = (a * CA[d] + bb * CB[d]) / p;
+        const double XPA = P[d] - CA[d], XPB = P[d] - CB[d];
+        for (int x = 0; x <= GTF_LMAX; x++)
+            for (int y = 0; y <= GTF_LMAX; y++)
+                for (int t = 0; t < 2 * GTF_LMAX + 2; t++) E1[d][x][y][t] = 0.0;
+        E1[d][0][0][0] = 1.0;
+        for (int x = 0; x < la; x++)
+            for (int t = 0; t <= x + 1; t++) {
+                double v = XPA * E1[d][x][0][t] + (t + 1) * E1[d][x][0][t + 1];
+                if (t > 0) v += E1[d][x][0][t - 1] / (2.0 * p);
+                E1[d][x + 1][0][t] = v;
             }
-            const double scale = b.coef[b.poff[A] + i] * b.coef[b.poff[B] + j] * exp(-mu * AB2) / p;
-            double* out = E + eoff[pid] + (long)k * esz;
-            double* outz = Ez + zoff[pid] + (long)k * herm_nnz(la, lb);
-            int nzpos = 0;
-            double emax = 0.0;
-            const int ca0 = la == 0 ? 0 : nherm(la - 1), cb0 = lb == 0 ? 0 : nherm(lb - 1);
-            for (int as = 0; as < na; as++)
-                for (int bs = 0; bs < nb; bs++)
-                    for (int h = 0; h < H; h++) {
-                        const int t = herm_t(h), u = herm_u(h), v = herm_v(h);
-                        double s = 0.0;
-                        for (int ca = 0; ca < ncart(la); ca++) {
-                            const double ta = sph_coef(la, as, ca);
-                            if (ta == 0.0) continue;
-                            const int ax = herm_t(ca0 + ca), ay = herm_u(ca0 + ca), az = herm_v(ca0 + ca);
-                            for (int cb = 0; cb < ncart(lb); cb++) {
-                                const double tb = sph_coef(lb, bs, cb);
-                                if (tb == 0.0) continue;
-                                const int bx = herm_t(cb0 + cb), by = herm_u(cb0 + cb), bz = herm_v(cb0 + cb);
-                                if (t > ax + bx || u > ay + by || v > az + bz) continue;
-                                s += ta * tb * E1[0][ax][bx][t] * E1[1][ay][by][u] * E1[2][az][bz][v];
-                            }
-                        }
-                        out[(as * nb + bs) * H + h] = scale * s;
-                        if (herm_nz(la, lb, as, bs, h)) outz[nzpos++] = scale * s;
-                        emax = fmax(emax, fabs(scale * s));
+        for (int y = 0; y < lb; y++)
+            for (int x = 0; x <= la; x++)
+                for (int t = 0; t <= x + y + 1; t++) {
+                    double v = XPB * E1[d][x][y][t] + (t + 1) * E1[d][x][y][t + 1];
+                    if (t > 0) v += E1[d][x][y][t - 1] / (2.0 * p);
+                    E1[d][x][y + 1][t] = v;
+                }
+    }
+    const double scale = b.coef[b.poff[A] + i] * b.coef[b.poff[B] + j] * exp(-mu * AB2) / p;
+    int nzpos = 0;
+    double emax = 0.0;
+    const int ca0 = la == 0 ? 0 : nherm(la - 1), cb0 = lb == 0 ? 0 : nherm(lb - 1);
+    for (int as = 0; as < na; as++)
+        for (int bs = 0; bs < nb; bs++)
+            for (int h = 0; h < H; h++) {
+                const int t = herm_t(h), u = herm_u(h), v = herm_v(h);
+                double s = 0.0;
+                for (int ca = 0; ca < ncart(la); ca++) {
+                    const double ta = sph_coef(la, as, ca);
+                    if (ta == 0.0) continue;
+                    const int ax = herm_t(ca0 + ca), ay = herm_u(ca0 + ca), az = herm_v(ca0 + ca);
+                    for (int cb = 0; cb < ncart(lb); cb++) {
+                        const double tb = sph_coef(lb, bs, cb);
+                        if (tb == 0.0) continue;
+                        const int bx = herm_t(cb0 + cb), by = herm_u(cb0 + cb), bz = herm_v(cb0 + cb);
+                        if (t > ax + bx || u > ay + by || v > az + bz) continue;
+                        s += ta * tb * E1[0][ax][bx][t] * E1[1][ay][by][u] * E1[2][az][bz][v];
                     }
-            if (emax >= prim_eps) {
-                prim[poff[pid] + k] = make_double4(p, P[0], P[1], P[2]);
-                k++;
+                }
+                emax = fmax(emax, fabs(scale * s));
+                if (out) {
+                    out[(as * nb + bs) * H + h] = scale * s;
+                    if (herm_nz(la, lb, as, bs, h)) outz[nzpos++] = scale * s;
+                }
             }
+    if (prim_out) *prim_out = make_double4(p, P[0], P[1], P[2]);
+    return emax;
+}
+
+__global__ __launch_bounds__(256) void build_pairs_kernel(BasisView b, int npairs, const int* __restrict__ s1,
+                                                          const int* __restrict__ s2, const int* __restrict__ poff,
+                                                          const long* __restrict__ eoff, double4* __restrict__ prim,
+                                                          double* __restrict__ E, const long* __restrict__ zoff,
+                                                          double* __restrict__ Ez, int* __restrict__ nprim_out,
+                                                          double prim_eps)
+{
+    const int pid = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (pid >= npairs) return;
+    const int A = s1[pid], B = s2[pid];
+    const int la = b.L[A], lb = b.L[B];
+    const int esz = nsph(la) * nsph(lb) * nherm(la + lb), ez = herm_nnz(la, lb);
+    const int nA = b.nprim[A], nB = b.nprim[B], nraw = nA * nB;
+    int k = 0;  // survivors so far (wave-uniform)
+    for (int c0 = 0; c0 < nraw; c0 += 64) {
+        const int r = c0 + lane;
+        const bool live = r < nraw;
+        const int i = live ? r / nB : 0, j = live ? r - (r / nB) * nB : 0;
+        const bool keep = live && pair_prim(b, A, B, i, j, nullptr, nullptr, nullptr) >= prim_eps;
+        const unsigned long long mask = __ballot(keep);
+        if (keep) {
+            const int slot = k + __popcll(mask & ((1ull << lane) - 1ull));
+            pair_prim(b, A, B, i, j, E + eoff[pid] + (long)slot * esz, Ez + zoff[pid] + (long)slot * ez,
+                      prim + poff[pid] + slot);
         }
-    nprim_out[pid] = k;
+        k += __popcll(mask);
+    }
+    if (lane == 0) nprim_out[pid] = k;
 }
 
 // Row k = (bra pair r, ket segment s).  exact[k] = number of kets j of the
@@ -175,8 +201,8 @@ int gtf_build_pairs(hipStream_t st, const BasisView& b, int npairs, const int* s
                     int* nprim_out, double prim_eps)
 {
     if (npairs <= 0) return 0;
-    build_pairs_kernel<<<(npairs + 127) / 128, 128, 0, st>>>(b, npairs, s1, s2, poff, eoff, prim, E, zoff, Ez,
-                                                           nprim_out, prim_eps);
+    build_pairs_kernel<<<(npairs + 3) / 4, 256, 0, st>>>(b, npairs, s1, s2, poff, eoff, prim, E, zoff, Ez, nprim_out,
+                                                       prim_eps);
     GTF_CHECK(hipGetLastError());
     return 0;
 }

@@ -477,7 +477,7 @@ int gtf_server_set_join_gap(gtf_server_t s, int iters)
 
 int gtf_server_set_tile_shapes(gtf_server_t s, const int* shapes)
 {
-    if (!s) return -3;
+    if (!s || (shapes && gtf_tile_shapes_check(shapes))) return -3;
     std::lock_guard<std::mutex> lk(s->mu);
     s->shape_ranges.clear();
     if (shapes)
@@ -487,7 +487,7 @@ int gtf_server_set_tile_shapes(gtf_server_t s, const int* shapes)
 
 int gtf_server_set_tile_shapes_range(gtf_server_t s, int nbf_min, int nbf_max, const int* shapes)
 {
-    if (!s || !shapes || nbf_min < 0 || nbf_max < nbf_min) return -3;
+    if (!s || !shapes || nbf_min < 0 || nbf_max < nbf_min || gtf_tile_shapes_check(shapes)) return -3;
     std::lock_guard<std::mutex> lk(s->mu);
     s->shape_ranges.push_back({nbf_min, nbf_max, std::vector<int>(shapes, shapes + 21 * GTF_TILE_SHAPE_FIELDS)});
     return 0;

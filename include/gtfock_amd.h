@@ -21,6 +21,19 @@ extern "C" {
  *           device bytes held} */
 int gtf_pfock_stats(PFock_t pfock, double *out);
 
+/* Densities the last PFock_computeFock digested from each integral in one
+ * pass (pfock/fock_task.c:46 loops num_dmat inside update_F): num_dmat
+ * when the per-density LDS work areas of every class fit, else 1 (one
+ * pass per density). */
+int gtf_pfock_fused_densities(PFock_t pfock);
+
+/* Load balance of the last PFock_computeFock (the figures
+ * PFock_getStatistics prints; pfock/pfock.c:1918-2010 gathers them across
+ * processes): out[0] = slowest persistent workgroup over the mean, summed
+ * over the tile launches; out[1] = tiles taken from a class counter after a
+ * workgroup's first; out[2] = workgroups.  Returns 0, or -1. */
+int gtf_pfock_balance(PFock_t pfock, double *out);
+
 /* The screening CSR in the layout of pfock/screening.c:152-175
  * (shellptr[nshells+1], shellid/shellrid/shellvalue[nnz]); returns nnz. */
 int gtf_pfock_csr(PFock_t pfock, int *shellptr, int *shellid, int *shellrid, double *shellvalue);
@@ -95,6 +108,9 @@ int gtf_pfock_set_tile_shape(PFock_t pfock, int cls, const int *shape);
 /* all 21 classes at once, shapes[21 * GTF_TILE_SHAPE_FIELDS] in class
  * order; nothing changes unless every shape is valid */
 int gtf_pfock_set_tile_shapes(PFock_t pfock, const int *shapes);
+/* 0 when every one of the 21 shapes is valid (positive sizes, 0/1 flags,
+ * exchanged only where the class has that kernel), else -3 */
+int gtf_tile_shapes_check(const int *shapes);
 
 /* Integrals of nq shell quartets (4 shell ids each) in the reference layout
  * iM + dM*(iN + dN*(iP + dP*iQ)), concatenated in input order. */
@@ -123,6 +139,12 @@ struct _purif_t;
 int gtf_purif_set_precision(struct _purif_t *purif, int mode, double until);
 /* slices per operand row of the split-BF16 products (3..6, default 5) */
 int gtf_purif_set_slices(struct _purif_t *purif, int ns);
+/* Mode 2 uses the split products only while the SCF's last DIIS residual
+ * ||X (F D S - S D F) X||_F (compute_diis) is at least resid (default
+ * 1e-3; before the SCF's first residual they are always used); closer to
+ * convergence every product is FP64, so the SCF converges onto the FP64
+ * fixed point.  0: split in every SCF iteration.  Returns 0 or -1. */
+int gtf_purif_set_split_residual(struct _purif_t *purif, double resid);
 /* purification iterations of the last compute_purification run in the
  * reduced precision */
 int gtf_purif_fp32_iters(struct _purif_t *purif);
@@ -152,6 +174,8 @@ int gtf_batch_set_precision(gtf_batch_t batch, double tau);
 int gtf_batch_set_purif_precision(gtf_batch_t batch, int mode, double until);
 /* slices of the split-BF16 products (3..6, default 5) */
 int gtf_batch_set_purif_slices(gtf_batch_t batch, int ns);
+/* per molecule, as gtf_purif_set_split_residual (default 1e-3) */
+int gtf_batch_set_purif_residual(gtf_batch_t batch, double resid);
 /* The tile autotuner on the batch's packed density, as gtf_pfock_autotune
  * (the next build re-captures its graph); shapes per class as
  * gtf_pfock_tile_shape / gtf_pfock_set_tile_shape, so the shapes tuned on

@@ -165,9 +165,9 @@ class Basis:
         D = np.ascontiguousarray(D, dtype=np.float64)
         J = np.zeros((n, n))
         K = np.zeros((n, n))
-        st = np.zeros(3)
+        st = np.zeros(5)
         lib().ob_fock_build(self.h, tol, nblocks, _p(D), _p(J), _p(K), _p(st))
-        return J, K, dict(usq=st[0], uitl=st[1], nnz=st[2])
+        return J, K, dict(usq=st[0], uitl=st[1], nnz=st[2], screen_s=st[3], build_s=st[4])
 
     def quartets(self, tol=1e-11):
         n = lib().ob_enumerate_quartets(self.h, tol, None, 0)

@@ -9,6 +9,7 @@
 
 #include <ctype.h>
 #include <math.h>
+#include <time.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -780,6 +781,13 @@ static void update_jk(const ob_basis *b, int M, int N, int P, int Q, int f1, int
                 }
 }
 
+static double wall_seconds(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
 static void symmetrise(int n, double *A)
 {
     for (int i = 0; i < n; i++)
@@ -794,7 +802,9 @@ void ob_fock_build(const ob_basis *b, double tol, int nblocks, const double *D,
 {
     int n = b->nbf, ns = b->nshells;
     csr_t c;
+    double t0 = wall_seconds();
     make_csr(b, tol, &c);
+    double t1 = wall_seconds();
     double tol2 = tol * tol;
     if (nblocks <= 0) nblocks = 4;
     if (nblocks > ns) nblocks = ns;
@@ -853,7 +863,11 @@ void ob_fock_build(const ob_basis *b, double tol, int nblocks, const double *D,
     free(Jt); free(Kt);
     symmetrise(n, Jacc);
     symmetrise(n, Kacc);
-    if (st) { st->usq = usq; st->uitl = uitl; st->nnz = c.nnz; }
+    if (st) {
+        st->usq = usq; st->uitl = uitl; st->nnz = c.nnz;
+        st->screen_s = t1 - t0;
+        st->build_s = wall_seconds() - t1;
+    }
     free(blk);
     free_csr(&c);
 }

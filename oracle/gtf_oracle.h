@@ -72,6 +72,8 @@ typedef struct {
     double usq;     /* shell quartets computed */
     double uitl;    /* integrals computed */
     double nnz;     /* significant shell pairs */
+    double screen_s; /* wall seconds of the screening (CSR) pass */
+    double build_s;  /* wall seconds of the rest: quartet loop, reduction, symmetrisation */
 } ob_fock_stats;
 
 /* J and K accumulators after symmetrisation: Jacc = 2J(D), Kacc = -K(D),

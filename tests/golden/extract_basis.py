@@ -7,7 +7,8 @@ element from the basis files that accompany GTFock (data/cc-pvdz.gbs and
 data/sto-3g.gbs), keeping only the elements the workloads use, so that the GPU
 box (which has no copy of GTFock) can load them.
 
-Run once, from this directory:  python extract_basis.py /root/reference/data
+Run once (it writes into gtfock_amd/data/):
+    python tests/golden/extract_basis.py /root/reference/data
 """
 import os
 import sys
@@ -41,6 +42,6 @@ def extract(src, dst, elements):
 
 if __name__ == "__main__":
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/data"
-    here = os.path.dirname(os.path.abspath(__file__))
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "..", "gtfock_amd", "data")
     extract(os.path.join(ref, "cc-pvdz.gbs"), os.path.join(here, "cc-pvdz.gbs"), ELEMENTS)
     extract(os.path.join(ref, "sto-3g.gbs"), os.path.join(here, "sto-3g.gbs"), ELEMENTS)

@@ -64,6 +64,20 @@ void gtf_batch_destroy(gtf_batch_t b)
 
 int gtf_batch_set_precision(gtf_batch_t, double) { return 0; }
 
+// the library's check without the per-class exchanged-kernel rule
+int gtf_tile_shapes_check(const int* shapes)
+{
+    if (!shapes) return -3;
+    for (int c = 0; c < 21; c++) {
+        const int* in = shapes + GTF_TILE_SHAPE_FIELDS * c;
+        for (int f = 0; f < 4; f++)
+            if (in[f] < 1) return -3;
+        for (int f = 4; f < GTF_TILE_SHAPE_FIELDS; f++)
+            if (in[f] < 0 || in[f] > 1) return -3;
+    }
+    return 0;
+}
+
 static std::atomic<int> g_shapes_set{0}, g_shape0_units{0};
 static std::vector<int> g_shape_units;  // class-0 units of every set, in order
 int gtf_batch_set_tile_shapes(gtf_batch_t, const int* shapes)

@@ -122,3 +122,52 @@ def test_h2_without_d_classes(g, gbs, mol_dir):
     m = min(len(e), len(eo))
     assert np.abs(e[:m] - eo[:m]).max() < 1e-9
     pf.close()
+
+
+@gpu
+def test_several_densities_in_one_pass(g, gbs, mol_dir):
+    """Three committed densities of C10H22 are digested from each integral in
+    one build (update_F's num_dmat loop); each result equals the build of
+    that density alone."""
+    L = g.lib()
+    b = g.BasisSet(gbs, str(mol_dir / "c10h22.xyz"))
+    n = b.nbf
+    Ds = [_rand_sym(n, 20 + k) for k in range(3)]
+    single = []
+    p1 = g.PFock(b)
+    for D in Ds:
+        single.append(p1.fock(D))
+    p1.close()
+    pf = g.PFock(b, max_numdmat=3)
+    L.PFock_setNumDenMat(3, pf.h)
+    for k, D in enumerate(Ds):
+        assert L.PFock_putDenMat(0, n - 1, 0, n - 1, n, _p(np.ascontiguousarray(D)), k, pf.h) == SUCCESS
+    assert L.PFock_commitDenMats(pf.h) == SUCCESS
+    assert L.PFock_computeFock(b.h, pf.h) == SUCCESS
+    assert L.gtf_pfock_fused_densities(pf.h) == 3
+    for k in range(3):
+        F = pf.get_mat(g.MAT_F, k)
+        err = np.abs(F - single[k]).max()
+        print("density %d: max |F_fused - F_alone| %.2e" % (k, err))
+        assert err < 1e-11
+    pf.close()
+
+
+@gpu
+def test_statistics_report_measured_balance(g, gbs, mol_dir, capfd):
+    """PFock_getStatistics (pfock/pfock.c:1918-2010) prints the load balance
+    and steals of the last build, measured per persistent workgroup."""
+    b = g.BasisSet(gbs, str(mol_dir / "c10h22.xyz"))
+    pf = g.PFock(b)
+    D = _rand_sym(b.nbf, 7)
+    pf.fock(D)
+    bal = pf.balance()
+    print(bal)
+    assert bal["workgroups"] > 0
+    assert 1.0 <= bal["load_balance"] < 4.0
+    assert bal["steals"] >= 0
+    capfd.readouterr()
+    assert g.lib().PFock_getStatistics(pf.h) == SUCCESS
+    out = capfd.readouterr().out
+    assert "load blance = %.3f" % bal["load_balance"] in out
+    pf.close()

@@ -2,8 +2,10 @@
 
 * C10H22 / cc-pVDZ (250 basis functions): the GPU's 15-iteration FP64 SCF
   against the oracle's energies (tests/golden/scf_c10h22_oracle.json,
-  written by tests/golden/make_oracle_scf.py), every iteration within
-  1e-9 Eh.
+  written by tests/golden/make_oracle_scf.py): the converged energy within
+  1e-9 Eh, and every iteration within 1e-9 Eh or, in the middle iterations
+  where the reference algorithm amplifies FP64 rounding, within ten times
+  the spread of the oracle's own reruns with other thread counts.
 * C54H18 / cc-pVDZ (846 basis functions): FP64 ERIs of sampled shell
   quartets against the oracle (1e-12); the mixed form -- FP32 ERIs below a
   Schwarz bound of 1e-4 and split-BF16 purification -- against the plain
@@ -19,7 +21,6 @@
   split-BF16 purification): every molecule converges in both forms, within
   1e-6 Eh of the FP64 batched SCF.
 """
-import json
 import os
 
 import numpy as np
@@ -39,15 +40,16 @@ def g():
 
 
 @gpu
-def test_c10h22_scf_matches_oracle_energies(g, gbs, mol_dir):
-    gold = json.load(open(os.path.join(HERE, "golden", "scf_c10h22_oracle.json")))
-    eo = np.array(gold["energies"])
+def test_c10h22_scf_matches_oracle_energies(g, gbs, mol_dir, c10h22_gold):
+    gold, eo, tol = c10h22_gold
     e, _, _ = g.scf(gbs, str(mol_dir / "c10h22.xyz"), gold["iterations"], tolscr=gold["tolscr"])
     n = min(len(e), len(eo))
     d = np.abs(e[:n] - eo[:n])
-    print("C10H22 SCF: GPU %.12f, oracle %.12f over %d iterations; max |dE| %.2e" % (e[n - 1], eo[n - 1], n, d.max()))
+    print("C10H22 SCF: GPU %.12f, oracle %.12f over %d iterations; |dE| per iteration %s; tolerance %s" % (
+        e[n - 1], eo[n - 1], n, " ".join("%.1e" % x for x in d), " ".join("%.0e" % x for x in tol)))
     assert n == len(eo)
-    assert d.max() < 1e-9
+    assert np.all(d <= tol[:n])
+    assert d[-1] < 1e-9
 
 
 @gpu

