@@ -182,6 +182,21 @@ def time_batch(bt, steps):
     return (time.perf_counter() - t0) / steps
 
 
+def scf_logged(bt, max_iters, ediff, tag):
+    """Batched SCF one iteration at a time (gtf_batch_scf_step) with a
+    progress line per iteration; returns (energies, builds) as Batch.scf."""
+    bt.scf_begin(max_iters=max_iters, ediff=ediff)
+    it = 0
+    while True:
+        left = bt.scf_step()
+        it += 1
+        log("%s: iteration %d, %d molecules still running" % (tag, it, left))
+        if left == 0:
+            break
+    e, n, _ = bt.scf_results()
+    return e, n
+
+
 def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters, tune_nmol, tune_reps, tiles):
     """Many molecules per build: the seeded alkane / water-cluster batch.
     Molecule-builds per second of (a) one PFock per molecule, FP64, built one
@@ -268,14 +283,15 @@ def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters, tune_nmol, tune_reps, t
         if tuned:
             b64.set_tile_shapes(tuned["fp64"][0])
         t0 = time.perf_counter()
-        e64, it64 = b64.scf(max_iters=scf_iters, ediff=1e-8)
+        e64, it64 = scf_logged(b64, scf_iters, 1e-8, "batched SCF FP64")
         t_scf64 = time.perf_counter() - t0
         b64.close()
         b32 = g.Batch(bases, tau=tau)
+        b32.set_purif_precision("split")
         if tuned:
             b32.set_tile_shapes(tuned["mixed"][0])
         t0 = time.perf_counter()
-        e32, it32 = b32.scf(max_iters=scf_iters, ediff=1e-8)
+        e32, it32 = scf_logged(b32, scf_iters, 1e-8, "batched SCF mixed")
         t_scf = time.perf_counter() - t0
         b32.close()
         # energies are compared where both forms converged; a molecule that
@@ -297,6 +313,8 @@ def batch_leg(g, gbs, nmol, seed, tau, steps, scf_iters, tune_nmol, tune_reps, t
             for key, t, sh in (("fp64", 0.0, tuned["fp64"][0] if tuned else None),
                                ("mixed", tau, tuned["mixed"][0] if tuned else None)):
                 bk = g.Batch([bases[k]], tau=t)
+                if t > 0:
+                    bk.set_purif_precision("split")
                 if sh is not None:
                     bk.set_tile_shapes(sh)
                 e, it = bk.scf(max_iters=scf_iters, ediff=1e-8)
@@ -385,10 +403,18 @@ def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max
            "p99_s": float(np.percentile(lat, 99)), "mean_builds": float(np.mean([r["iters"] for r in ok])),
            "not_converged": int(sum(1 for r in ok if r["iters"] >= max_iters or r["iters"] < 0))}
     if ref:
-        log("serve: FP64 reference batch of the %d molecules" % njobs)
-        bt = g.Batch(bases)
-        e64, it64 = bt.scf(max_iters=max_iters, ediff=1e-8)
+        # one FP64 SCF per distinct molecule (the stream repeats molecules)
+        first = {}
+        for k, (name, _) in enumerate(mols):
+            first.setdefault(name, k)
+        keys = sorted(first.values())
+        log("serve: FP64 reference batch of the %d distinct molecules" % len(keys))
+        bt = g.Batch([bases[k] for k in keys])
+        e64u, it64u = scf_logged(bt, max_iters, 1e-8, "serve FP64 reference")
         bt.close()
+        ref64 = {mols[k][0]: (e64u[i], it64u[i]) for i, k in enumerate(keys)}
+        e64 = [ref64[name][0] for name, _ in mols]
+        it64 = [ref64[name][1] for name, _ in mols]
         errs = [abs(r["energy"] - e64[k]) for k, r in enumerate(res)
                 if r["status"] == 2 and 0 < r["iters"] < max_iters and 0 < it64[k] < max_iters]
         out["max_abs_dE_vs_fp64_Eh"] = float(max(errs)) if errs else None
@@ -459,10 +485,12 @@ def big_leg(g, gbs, name, tau, steps, scf_iters, tune, purif_reps=0):
         t0 = time.perf_counter()
         Dm, it = pu.purify(Fo)
         dt = time.perf_counter() - t0
-        pur[mode] = {"seconds": dt, "iterations": it, "low_iterations": pu.fp32_iters()}
+        pur[mode] = {"seconds": dt, "gpu_seconds": pu.gpu_seconds(), "iterations": it,
+                     "low_iterations": pu.fp32_iters()}
         Ds[mode] = Dm
         pu.close()
     pur["split_speedup"] = pur["fp64"]["seconds"] / pur["split"]["seconds"]
+    pur["split_gpu_speedup"] = pur["fp64"]["gpu_seconds"] / max(1e-12, pur["split"]["gpu_seconds"])
     pur["max_abs_dD"] = float(np.abs(Ds["split"] - Ds["fp64"]).max())
     for mode in ("fp64", "split") if purif_reps > 0 else ():
         # further purifications of the same matrix, for kernel profiles
@@ -549,7 +577,7 @@ def main():
                     help="directory to write the shapes tuned in this run to, as tuning tables")
     ap.add_argument("--batch-seed", type=int, default=20261202)
     ap.add_argument("--batch-scf", type=int, default=30, help="SCF iterations of the batched energy check (0: skip)")
-    ap.add_argument("--serve", type=int, default=48, help="jobs of the serving leg (0: skip)")
+    ap.add_argument("--serve", type=int, default=32, help="jobs of the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--serve-rings", type=int, default=4,
                     help="largest graphene flake (rings) in the request stream (4: C96H24, 1464 basis functions)")

@@ -147,6 +147,7 @@ def lib():
         "gtf_purif_set_split_residual": ([_vp, ctypes.c_double], ctypes.c_int),
         "gtf_batch_set_purif_residual": ([_vp, ctypes.c_double], ctypes.c_int),
         "gtf_tile_shapes_check": ([_ip], ctypes.c_int),
+        "gtf_batch_job_bytes": ([_vp], ctypes.c_double),
         "gtf_gemm_split_host": ([ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "gtf_batch_set_purif_precision": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "gtf_batch_set_purif_slices": ([_vp, ctypes.c_int], ctypes.c_int),
@@ -533,6 +534,10 @@ class Purif:
         """Split products only while the SCF's last DIIS residual is at least
         resid (gtf_purif_set_split_residual; 0: in every SCF iteration)."""
         _check(lib().gtf_purif_set_split_residual(self.h, float(resid)), "gtf_purif_set_split_residual")
+
+    def gpu_seconds(self):
+        """device time of the last purification (its loop and X D X, HIP events)"""
+        return float(self._fields().timedgemm)
 
     def fp32_iters(self):
         """iterations of the last purification run in the reduced precision"""

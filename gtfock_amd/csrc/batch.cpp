@@ -1054,7 +1054,7 @@ double gtf_batch_job_bytes(BasisSet_t basis)
     // + the split purification's BF16 slices of both operands at the most
     // slices (and their exponents and row maxima)
     const double split = 2.0 * 2.0 * OZ_SLICES_MAX * pl * pl + 4.0 * 8.0 * pl * pl / 64.0;
-    return pairs * 1.25 + 8.0 * (9.0 * n * n + (8.0 + 2.0 * B_MAX_DIIS) * pl * pl) + split + (1 << 20);
+    return pairs * 1.6 + 8.0 * (9.0 * n * n + (8.0 + 2.0 * B_MAX_DIIS) * pl * pl) + split + (1 << 20);
 }
 
 int gtf_batch_get_density(gtf_batch_t b, int m, double* D)

@@ -193,6 +193,9 @@ void GtfSystem::dfree(void* p)
     if (!p) return;
     auto it = std::find(allocs.begin(), allocs.end(), p);
     if (it == allocs.end()) return;
+    const size_t k = it - allocs.begin();
+    live_bytes -= alloc_bytes[k];
+    alloc_bytes.erase(alloc_bytes.begin() + k);
     allocs.erase(it);
     (void)hipFree(p);
 }
@@ -1044,5 +1047,6 @@ void GtfSystem::release()
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
     for (void* p : allocs) (void)hipFree(p);
     allocs.clear();
-    dev_bytes = 0;
+    alloc_bytes.clear();
+    live_bytes = dev_bytes = 0;
 }

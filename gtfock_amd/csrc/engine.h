@@ -166,8 +166,10 @@ struct GtfSystem {
     {
         return balance && d_wgstat && grid == wg_n[k] ? d_wgstat + 3 * wg_off[k] : nullptr;
     }
-    size_t dev_bytes = 0;
+    // device bytes the system holds now, and the most it held at once
+    size_t live_bytes = 0, dev_bytes = 0;
     std::vector<void*> allocs;
+    std::vector<size_t> alloc_bytes;  // beside allocs
 
     int init(const std::vector<const BasisSet*>& mols, hipStream_t st);
     int init(const BasisSet* b, hipStream_t st) { return init(std::vector<const BasisSet*>{b}, st); }
@@ -241,7 +243,9 @@ struct GtfSystem {
         if (n == 0) n = 1;
         if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
         allocs.push_back(p);
-        dev_bytes += n * sizeof(T);
+        alloc_bytes.push_back(n * sizeof(T));
+        live_bytes += n * sizeof(T);
+        dev_bytes = std::max(dev_bytes, live_bytes);
         return (T*)p;
     }
 };

@@ -14,11 +14,11 @@
 //
 // Products.  C[m][n] = 2^(eA_m + eB_n - 14) sum_d 2^(-7 d) S_d[m][n] with
 //     S_d = sum_{i + j = d, i, j < NS} QA_i * QBt_j^T,   d < NS.
-// Every S_d is a sum of integer products; over a K chunk of 256 its partial
-// sums stay below 2^23 (|q_0 q_0| <= 2^14, |q_0 q_j| <= 2^13, |q_i q_j| <= 2^12,
-// at most NS - 1 <= 5 pairs per d), so the MFMA's FP32 accumulation is exact.
-// After each chunk the FP32 sums are added, scaled by 2^(-7 d), into an FP64
-// accumulator.  The only error is the slicing residual and the dropped
+// Every S_d is a sum of integer products; over a K chunk of at most 256 its
+// partial sums stay below 2^23 (|q_0 q_0| <= 2^14, |q_0 q_j| <= 2^13,
+// |q_i q_j| <= 2^12, at most NS - 1 <= 5 pairs per d), so the MFMA's FP32
+// accumulation is exact.  After each chunk (96 of K here) the FP32 sums are
+// added, scaled by 2^(-7 d), into an FP64 accumulator.  The only error is the slicing residual and the dropped
 // pairs with i + j >= NS: about 2^(-7 NS) relative to the row maxima.
 //
 // Tile: a 256-thread workgroup computes a 64 x 64 block of C; wave w owns the
@@ -39,8 +39,7 @@ typedef float oz_f32x16_t __attribute__((ext_vector_type(16)));
 
 #define OZ_MAX_SLICES 6
 #define OZ_KB 32       // K depth staged in LDS per step
-#define OZ_KCHUNK 256  // K depth between FP32 -> FP64 flushes
-#define OZ_LDS_ROW 40  // BF16 per LDS row (32 + 8 padding)
+#define OZ_KCHUNK 256  // most K depth between FP32 -> FP64 flushes
 
 // One molecule's operand: base pointer and leading dimension.
 struct OzView {
@@ -135,80 +134,124 @@ __device__ __forceinline__ void oz_split_tile(int ld, const double* __restrict__
     }
 }
 
+typedef __attribute__((address_space(3))) void oz_lds_void;
+
+// One K block of a 64 x 64 tile in LDS: NS slices of 64 operand rows x
+// OZ_KB, 64 B per row, the four 16-B chunks of row r stored at chunk
+// position (chunk ^ (r & 3)) so that the fragment reads of 32 rows spread
+// over the banks.
+template <int NS>
+struct OzStage {
+    __bf16 a[NS][64][OZ_KB];
+    __bf16 b[NS][64][OZ_KB];
+};
+
 // 64 x 64 block of C from NS slices of A (rows) and of B^T (rows = columns of
-// C), scaled back by the row exponents.
+// C), scaled back by the row exponents.  The K blocks stream through three
+// LDS stages filled by direct global -> LDS loads (global_load_lds_dwordx4),
+// issued two blocks ahead of the block being multiplied, so that the load
+// latency of the slices (read once per tile, mostly from the last-level
+// cache) hides behind MFMA work with one wave per SIMD and no registers
+// spent on staging.
 template <int NS>
 __device__ __forceinline__ void oz_gemm_tile(int ld, const __bf16* __restrict__ QA, const int* __restrict__ EA,
                                              const __bf16* __restrict__ QB, const int* __restrict__ EB,
                                              double* __restrict__ C, int row0, int col0)
 {
-    __shared__ __align__(16) __bf16 As[NS][64][OZ_LDS_ROW];
-    __shared__ __align__(16) __bf16 Bs[NS][64][OZ_LDS_ROW];
+    __shared__ __align__(16) OzStage<NS> st0, st1, st2;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const long sl = (long)ld * ld;
+    const int nblk = ld / OZ_KB;
     oz_f32x16_t acc[NS];
     double acc64[16];
     for (int r = 0; r < 16; r++) acc64[r] = 0.0;
     for (int d = 0; d < NS; d++)
         for (int r = 0; r < 16; r++) acc[d][r] = 0.f;
-    // global -> LDS: thread t copies 8 BF16 of row t / 4 at k offset 8 (t % 4)
-    const int lr = tid >> 2, lk = (tid & 3) * 8;
-    const __bf16* ga = QA + (long)(row0 + lr) * ld + lk;
-    const __bf16* gb = QB + (long)(col0 + lr) * ld + lk;
-    // the next K block's slices are loaded into registers while the current
-    // one is multiplied out of LDS
-    uint4 va[NS], vb[NS];
-    auto load = [&](int k0) {
+    // loads: wave w fills rows 16 w .. 16 w + 15 of every slice of both
+    // operands; lane l writes LDS chunk position l % 4 of row 16 w + l / 4
+    const int lrow = wave * 16 + (lane >> 2);
+    const int lchunk = (lane & 3) ^ (lrow & 3);
+    const __bf16* ga = QA + (long)(row0 + lrow) * ld + 8 * lchunk;
+    const __bf16* gb = QB + (long)(col0 + lrow) * ld + 8 * lchunk;
+    auto issue = [&](OzStage<NS>& S, int k0) {
         for (int i = 0; i < NS; i++) {
-            va[i] = *reinterpret_cast<const uint4*>(ga + i * sl + k0);
-            vb[i] = *reinterpret_cast<const uint4*>(gb + i * sl + k0);
+            __builtin_amdgcn_global_load_lds(ga + i * sl + k0, (oz_lds_void*)&S.a[i][wave * 16][0], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(gb + i * sl + k0, (oz_lds_void*)&S.b[i][wave * 16][0], 16, 0, 0);
         }
     };
-    auto store = [&]() {
-        for (int i = 0; i < NS; i++) {
-            *reinterpret_cast<uint4*>(&As[i][lr][lk]) = va[i];
-            *reinterpret_cast<uint4*>(&Bs[i][lr][lk]) = vb[i];
-        }
-    };
-    load(0);
-    store();
-    __syncthreads();
-    int kc = 0;
-    for (int k0 = 0; k0 < ld; k0 += OZ_KB) {
-        if (k0 + OZ_KB < ld) load(k0 + OZ_KB);
+    // fragment rows of this wave's 32 x 32 quadrant
+    const int ra = wr * 32 + (lane & 31), rb = wc * 32 + (lane & 31);
+    auto compute = [&](const OzStage<NS>& S) {
         for (int kk = 0; kk < OZ_KB; kk += 16) {
+            const int c = (kk >> 3) + (lane >> 5);  // logical 16-B chunk of the k range
             oz_bf16x8_t a[NS], b[NS];
             for (int i = 0; i < NS; i++) {
-                a[i] = *reinterpret_cast<const oz_bf16x8_t*>(&As[i][wr * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
-                b[i] = *reinterpret_cast<const oz_bf16x8_t*>(&Bs[i][wc * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
+                a[i] = *reinterpret_cast<const oz_bf16x8_t*>(&S.a[i][ra][8 * (c ^ (ra & 3))]);
+                b[i] = *reinterpret_cast<const oz_bf16x8_t*>(&S.b[i][rb][8 * (c ^ (rb & 3))]);
             }
-            static_for<0, NS>([&](auto d_) {
-                constexpr int d = decltype(d_)::value;
-                static_for<0, d + 1>([&](auto i_) {
-                    constexpr int i = decltype(i_)::value;
-                    acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[d - i], acc[d], 0, 0, 0);
+            // slice pair (i, j) adds into S_(i+j); i outermost, so that
+            // consecutive MFMAs update different accumulators; the FP32 sums
+            // are exact, so the order does not change them
+            static_for<0, NS>([&](auto i_) {
+                constexpr int i = decltype(i_)::value;
+                static_for<0, NS - i>([&](auto j_) {
+                    constexpr int j = decltype(j_)::value;
+                    acc[i + j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i + j], 0, 0, 0);
                 });
             });
         }
-        __syncthreads();  // every wave is done with this block
-        if (k0 + OZ_KB < ld) {
-            store();
-            __syncthreads();
+    };
+    auto flush = [&]() {
+        for (int r = 0; r < 16; r++) {
+            double s = 0.0;
+            for (int d = NS - 1; d >= 0; d--) s = s * (1.0 / 128.0) + (double)acc[d][r];
+            acc64[r] += s;
         }
-        kc += OZ_KB;
-        if (kc == OZ_KCHUNK || k0 + OZ_KB >= ld) {
-            for (int r = 0; r < 16; r++) {
-                double s = 0.0;
-                for (int d = NS - 1; d >= 0; d--) s = s * (1.0 / 128.0) + (double)acc[d][r];
-                acc64[r] += s;
-            }
-            for (int d = 0; d < NS; d++)
-                for (int r = 0; r < 16; r++) acc[d][r] = 0.f;
-            kc = 0;
+        for (int d = 0; d < NS; d++)
+            for (int r = 0; r < 16; r++) acc[d][r] = 0.f;
+    };
+    // block kb from stage cur; block kb + 2 loads into stage nxt (the one
+    // block kb - 1 used).  __syncthreads: every wave's loads have landed (it
+    // waits for all of them, block kb + 1's included) and every wave's LDS
+    // reads of the previous step are done before any wave refills that
+    // step's stage; a bare s_barrier would let the compiler move LDS reads
+    // across it.
+    auto step = [&](const OzStage<NS>& cur, OzStage<NS>& nxt, int kb) {
+        __syncthreads();
+        issue(nxt, (kb + 2) * OZ_KB);
+        compute(cur);
+    };
+    // the last blocks, with fewer (or no) loads ahead
+    auto step_tail = [&](const OzStage<NS>& cur, OzStage<NS>& nxt, int kb) {
+        __syncthreads();
+        if (kb + 2 < nblk) issue(nxt, (kb + 2) * OZ_KB);
+        compute(cur);
+    };
+    issue(st0, 0);
+    if (nblk > 1) issue(st1, OZ_KB);
+    // three blocks (96 of K) per pass, the stages in a fixed rotation, the
+    // FP32 sums flushed after each pass (exact: below OZ_KCHUNK); the first
+    // pass is peeled and at most four blocks are left for the tail
+    int kb = 0;
+    if (nblk >= 5) {
+        step(st0, st2, 0);
+        step(st1, st0, 1);
+        step(st2, st1, 2);
+        flush();
+        for (kb = 3; kb + 5 <= nblk; kb += 3) {
+            step(st0, st2, kb);
+            step(st1, st0, kb + 1);
+            step(st2, st1, kb + 2);
+            flush();
         }
     }
+    if (kb < nblk) step_tail(st0, st2, kb);
+    if (kb + 1 < nblk) step_tail(st1, st0, kb + 1);
+    if (kb + 2 < nblk) step_tail(st2, st1, kb + 2);
+    if (kb + 3 < nblk) step_tail(st0, st2, kb + 3);
+    flush();
     const int col = col0 + wc * 32 + (lane & 31);
     const int eb = EB[col];
     for (int r = 0; r < 16; r++) {

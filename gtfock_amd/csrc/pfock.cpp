@@ -16,6 +16,7 @@
 #include "../../include/gtfock_amd.h"
 #include "../../include/pfock.h"
 #include "engine.h"
+#include "hostcopy.h"
 #include "oneel.h"
 #include "screen.h"
 
@@ -26,6 +27,7 @@ struct PFock {
     BasisSet* basis = nullptr;
     GtfSystem sys;
     hipStream_t stream = nullptr;
+    PinnedStage stage;  // host blocks in and out
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int nbf = 0, nshells = 0, natoms = 0, maxnfuncs = 0;
     int nprow = 1, npcol = 1, nbp_p = 4;
@@ -211,11 +213,8 @@ PFockStatus_t PFock_putDenMat(int rowstart, int rowend, int colstart, int colend
         return PFOCK_STATUS_INVALID_VALUE;
     }
     double* dst = pf->d_D[index] + (long)rowstart * pf->nbf + colstart;
-    PF_HIP(hipMemcpy2DAsync(dst, sizeof(double) * pf->nbf, dmat, sizeof(double) * stride,
-                            sizeof(double) * (colend - colstart + 1), rowend - rowstart + 1, hipMemcpyHostToDevice,
-                            pf->stream),
-           "hipMemcpy2DAsync");
-    PF_HIP(hipStreamSynchronize(pf->stream), "hipStreamSynchronize");
+    PF_HIP(stage_h2d(pf->stage, pf->stream, dst, pf->nbf, dmat, stride, rowend - rowstart + 1, colend - colstart + 1),
+           "stage_h2d");
     return PFOCK_STATUS_SUCCESS;
 }
 
@@ -283,11 +282,9 @@ PFockStatus_t PFock_getMat(PFock_t pf, PFockMatType_t type, int index, int rowst
     PFockStatus_t st;
     const double* src = matrix_ptr(pf, type, index, &st);
     if (st != PFOCK_STATUS_SUCCESS) return st;
-    PF_HIP(hipMemcpy2DAsync(mat, sizeof(double) * stride, src + (long)rowstart * pf->nbf + colstart,
-                            sizeof(double) * pf->nbf, sizeof(double) * (colend - colstart + 1),
-                            rowend - rowstart + 1, hipMemcpyDeviceToHost, pf->stream),
-           "hipMemcpy2DAsync");
-    PF_HIP(hipStreamSynchronize(pf->stream), "hipStreamSynchronize");
+    PF_HIP(stage_d2h(pf->stage, pf->stream, mat, stride, src + (long)rowstart * pf->nbf + colstart, pf->nbf,
+                     rowend - rowstart + 1, colend - colstart + 1),
+           "stage_d2h");
     return PFOCK_STATUS_SUCCESS;
 }
 
@@ -418,10 +415,9 @@ static PFockStatus_t get_block(PFock_t pf, const double* src, int rs, int re, in
         GTF_PRINTF("Invalid block\n");
         return PFOCK_STATUS_INVALID_VALUE;
     }
-    PF_HIP(hipMemcpy2DAsync(mat, sizeof(double) * stride, src + (long)rs * pf->nbf + cs, sizeof(double) * pf->nbf,
-                            sizeof(double) * (ce - cs + 1), re - rs + 1, hipMemcpyDeviceToHost, pf->stream),
-           "hipMemcpy2DAsync");
-    PF_HIP(hipStreamSynchronize(pf->stream), "hipStreamSynchronize");
+    PF_HIP(stage_d2h(pf->stage, pf->stream, mat, stride, src + (long)rs * pf->nbf + cs, pf->nbf, re - rs + 1,
+                     ce - cs + 1),
+           "stage_d2h");
     return PFOCK_STATUS_SUCCESS;
 }
 

@@ -16,6 +16,7 @@
 
 #include "../../include/purif.h"
 #include "engine.h"
+#include "hostcopy.h"
 #include "batch_linalg.h"
 #include "linalg.h"
 #include "ozaki_launch.h"
@@ -24,6 +25,7 @@
 struct purif_dev {
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    PinnedStage stage;  // F, D in and out
     int n = 0, ld = 0;
     double *F = nullptr, *D = nullptr, *D2 = nullptr, *D3 = nullptr, *X = nullptr, *S = nullptr;
     double *W = nullptr, *W2 = nullptr, *evecs = nullptr, *Fvecs = nullptr;
@@ -74,16 +76,13 @@ static double wtime()
 
 static int h2d(purif_dev* d, double* dst, const double* src, int ldx)
 {
-    PD_CHECK(hipMemcpy2DAsync(dst, sizeof(double) * d->ld, src, sizeof(double) * ldx, sizeof(double) * d->n, d->n,
-                              hipMemcpyHostToDevice, d->stream));
+    PD_CHECK(stage_h2d(d->stage, d->stream, dst, d->ld, src, ldx, d->n, d->n));
     return 0;
 }
 
 static int d2h(purif_dev* d, double* dst, const double* src, int ldx)
 {
-    PD_CHECK(hipMemcpy2DAsync(dst, sizeof(double) * ldx, src, sizeof(double) * d->ld, sizeof(double) * d->n, d->n,
-                              hipMemcpyDeviceToHost, d->stream));
-    PD_CHECK(hipStreamSynchronize(d->stream));
+    PD_CHECK(stage_d2h(d->stage, d->stream, dst, ldx, src, d->ld, d->n, d->n));
     return 0;
 }
 

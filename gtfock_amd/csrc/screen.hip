@@ -16,107 +16,134 @@
 // survivors are packed at the front of the pair's slots, in primitive order,
 // and nprim_out receives their count.
 //
-// One wavefront per shell pair, one lane per primitive pair (chunks of 64):
-// a first pass computes each primitive pair's largest |E|, a ballot gives
-// every survivor its packed slot, and a second pass writes its coefficients
-// there.
+// One 64-lane workgroup per shell pair.  Per shell pair the lanes tabulate,
+// in LDS, the Hermite index triples, the cartesian terms of every spherical
+// component pair and the compact position of every structurally nonzero
+// (component pair, Hermite index) entry.  Then for each primitive pair in
+// turn three lanes run the one-dimensional recursions E^x, E^y, E^z into LDS
+// and all lanes form the coefficient block from them, one entry per lane;
+// a wavefront max decides whether the primitive pair is kept.
+#define BP_MAXAB (nsph(GTF_LMAX) * nsph(GTF_LMAX))
+#define BP_MAXH nherm(2 * GTF_LMAX)
+#define BP_MAXTERM 9  // cartesian pairs of one spherical pair (d: at most 3 x 3)
+#define BP_NT (2 * GTF_LMAX + 2)
 
-// Hermite coefficients of primitive pair (i, j) of shells A, B: returns the
-// largest scaled |E|; with out != null also writes (p, P), the full and the
-// compact coefficient blocks (out, outz, prim_out)
-__device__ double pair_prim(const BasisView& b, int A, int B, int i, int j, double* out, double* outz,
-                            double4* prim_out)
+__global__ __launch_bounds__(64) void build_pairs_kernel(BasisView b, int npairs, const int* __restrict__ s1,
+                                                         const int* __restrict__ s2, const int* __restrict__ poff,
+                                                         const long* __restrict__ eoff, double4* __restrict__ prim,
+                                                         double* __restrict__ E, const long* __restrict__ zoff,
+                                                         double* __restrict__ Ez, int* __restrict__ nprim_out,
+                                                         double prim_eps)
 {
-    const int la = b.L[A], lb = b.L[B], lab = la + lb;
-    const int na = nsph(la), nb = nsph(lb), H = nherm(lab);
-    const double* CA = b.center + 3 * A;
-    const double* CB = b.center + 3 * B;
-    double AB2 = 0.0;
-    for (int d = 0; d < 3; d++) AB2 += (CA[d] - CB[d]) * (CA[d] - CB[d]);
-    const double a = b.alpha[b.poff[A] + i], bb = b.alpha[b.poff[B] + j];
-    const double p = a + bb, mu = a * bb / p;
-    double P[3];
-    double E1[3][GTF_LMAX + 1][GTF_LMAX + 1][2 * GTF_LMAX + 2];
-    for (int d = 0; d < 3; d++) {
-        P[d] = (a * CA[d] + bb * CB[d]) / p;
-        const double XPA = P[d] - CA[d], XPB = P[d] - CB[d];
-        for (int x = 0; x <= GTF_LMAX; x++)
-            for (int y = 0; y <= GTF_LMAX; y++)
-                for (int t = 0; t < 2 * GTF_LMAX + 2; t++) E1[d][x][y][t] = 0.0;
-        E1[d][0][0][0] = 1.0;
-        for (int x = 0; x < la; x++)
-            for (int t = 0; t <= x + 1; t++) {
-                double v = XPA * E1[d][x][0][t] + (t + 1) * E1[d][x][0][t + 1];
-                if (t > 0) v += E1[d][x][0][t - 1] / (2.0 * p);
-                E1[d][x + 1][0][t] = v;
-            }
-        for (int y = 0; y < lb; y++)
-            for (int x = 0; x <= la; x++)
-                for (int t = 0; t <= x + y + 1; t++) {
-                    double v = XPB * E1[d][x][y][t] + (t + 1) * E1[d][x][y][t + 1];
-                    if (t > 0) v += E1[d][x][y][t - 1] / (2.0 * p);
-                    E1[d][x][y + 1][t] = v;
-                }
-    }
-    const double scale = b.coef[b.poff[A] + i] * b.coef[b.poff[B] + j] * exp(-mu * AB2) / p;
-    int nzpos = 0;
-    double emax = 0.0;
-    const int ca0 = la == 0 ? 0 : nherm(la - 1), cb0 = lb == 0 ? 0 : nherm(lb - 1);
-    for (int as = 0; as < na; as++)
-        for (int bs = 0; bs < nb; bs++)
-            for (int h = 0; h < H; h++) {
-                const int t = herm_t(h), u = herm_u(h), v = herm_v(h);
-                double s = 0.0;
-                for (int ca = 0; ca < ncart(la); ca++) {
-                    const double ta = sph_coef(la, as, ca);
-                    if (ta == 0.0) continue;
-                    const int ax = herm_t(ca0 + ca), ay = herm_u(ca0 + ca), az = herm_v(ca0 + ca);
-                    for (int cb = 0; cb < ncart(lb); cb++) {
-                        const double tb = sph_coef(lb, bs, cb);
-                        if (tb == 0.0) continue;
-                        const int bx = herm_t(cb0 + cb), by = herm_u(cb0 + cb), bz = herm_v(cb0 + cb);
-                        if (t > ax + bx || u > ay + by || v > az + bz) continue;
-                        s += ta * tb * E1[0][ax][bx][t] * E1[1][ay][by][u] * E1[2][az][bz][v];
-                    }
-                }
-                emax = fmax(emax, fabs(scale * s));
-                if (out) {
-                    out[(as * nb + bs) * H + h] = scale * s;
-                    if (herm_nz(la, lb, as, bs, h)) outz[nzpos++] = scale * s;
-                }
-            }
-    if (prim_out) *prim_out = make_double4(p, P[0], P[1], P[2]);
-    return emax;
-}
-
-__global__ __launch_bounds__(256) void build_pairs_kernel(BasisView b, int npairs, const int* __restrict__ s1,
-                                                          const int* __restrict__ s2, const int* __restrict__ poff,
-                                                          const long* __restrict__ eoff, double4* __restrict__ prim,
-                                                          double* __restrict__ E, const long* __restrict__ zoff,
-                                                          double* __restrict__ Ez, int* __restrict__ nprim_out,
-                                                          double prim_eps)
-{
-    const int pid = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (pid >= npairs) return;
+    __shared__ double e1[3][GTF_LMAX + 1][GTF_LMAX + 1][BP_NT];
+    __shared__ double ent[BP_MAXAB * BP_MAXH];
+    __shared__ int zpos[BP_MAXAB * BP_MAXH];
+    __shared__ int hcode[BP_MAXH];
+    __shared__ double tcoef[BP_MAXAB][BP_MAXTERM];
+    __shared__ int tcode[BP_MAXAB][BP_MAXTERM];
+    __shared__ int nterm[BP_MAXAB];
+    __shared__ double pp[5];  // p, P, scale of the current primitive pair
+    const int pid = blockIdx.x, lane = threadIdx.x;
     const int A = s1[pid], B = s2[pid];
     const int la = b.L[A], lb = b.L[B];
-    const int esz = nsph(la) * nsph(lb) * nherm(la + lb), ez = herm_nnz(la, lb);
+    const int na = nsph(la), nb = nsph(lb), H = nherm(la + lb), nab = na * nb, esz = nab * H;
     const int nA = b.nprim[A], nB = b.nprim[B], nraw = nA * nB;
-    int k = 0;  // survivors so far (wave-uniform)
-    for (int c0 = 0; c0 < nraw; c0 += 64) {
-        const int r = c0 + lane;
-        const bool live = r < nraw;
-        const int i = live ? r / nB : 0, j = live ? r - (r / nB) * nB : 0;
-        const bool keep = live && pair_prim(b, A, B, i, j, nullptr, nullptr, nullptr) >= prim_eps;
-        const unsigned long long mask = __ballot(keep);
-        if (keep) {
-            const int slot = k + __popcll(mask & ((1ull << lane) - 1ull));
-            pair_prim(b, A, B, i, j, E + eoff[pid] + (long)slot * esz, Ez + zoff[pid] + (long)slot * ez,
-                      prim + poff[pid] + slot);
+    // ---- per shell pair tables
+    for (int h = lane; h < H; h += 64) hcode[h] = herm_t(h) | herm_u(h) << 4 | herm_v(h) << 8;
+    const int ca0 = la == 0 ? 0 : nherm(la - 1), cb0 = lb == 0 ? 0 : nherm(lb - 1);
+    for (int ab = lane; ab < nab; ab += 64) {
+        const int as = ab / nb, bs = ab - (ab / nb) * nb;
+        int n = 0;
+        for (int ca = 0; ca < ncart(la); ca++) {
+            const double ta = sph_coef(la, as, ca);
+            if (ta == 0.0) continue;
+            for (int cb = 0; cb < ncart(lb); cb++) {
+                const double tb = sph_coef(lb, bs, cb);
+                if (tb == 0.0) continue;
+                tcoef[ab][n] = ta * tb;
+                tcode[ab][n] = herm_t(ca0 + ca) | herm_u(ca0 + ca) << 2 | herm_v(ca0 + ca) << 4 |
+                               herm_t(cb0 + cb) << 6 | herm_u(cb0 + cb) << 8 | herm_v(cb0 + cb) << 10;
+                n++;
+            }
         }
-        k += __popcll(mask);
+        nterm[ab] = n;
     }
-    if (lane == 0) nprim_out[pid] = k;
+    for (int k = lane; k < 3 * (GTF_LMAX + 1) * (GTF_LMAX + 1) * BP_NT; k += 64) (&e1[0][0][0][0])[k] = 0.0;
+    // structural nonzeros in (component pair, Hermite index) order
+    int ez = 0;
+    for (int c0 = 0; c0 < esz; c0 += 64) {
+        const int idx = c0 + lane;
+        const bool nz = idx < esz && herm_nz(la, lb, (idx / H) / nb, (idx / H) % nb, idx % H);
+        const unsigned long long m = __ballot(nz);
+        if (idx < esz) zpos[idx] = nz ? ez + __popcll(m & ((1ull << lane) - 1ull)) : -1;
+        ez += __popcll(m);
+    }
+    const double* CA = b.center + 3 * A;
+    const double* CB = b.center + 3 * B;
+    const double AB2 = (CA[0] - CB[0]) * (CA[0] - CB[0]) + (CA[1] - CB[1]) * (CA[1] - CB[1]) +
+                       (CA[2] - CB[2]) * (CA[2] - CB[2]);
+    __syncthreads();
+    // ---- primitive pairs
+    int kept = 0;
+    for (int r = 0; r < nraw; r++) {
+        const int i = r / nB, j = r - (r / nB) * nB;
+        const double a = b.alpha[b.poff[A] + i], bb = b.alpha[b.poff[B] + j];
+        const double p = a + bb;
+        if (lane < 3) {
+            const int d = lane;
+            const double P = (a * CA[d] + bb * CB[d]) / p;
+            const double XPA = P - CA[d], XPB = P - CB[d];
+            double(*e)[GTF_LMAX + 1][BP_NT] = e1[d];
+            e[0][0][0] = 1.0;
+            for (int x = 0; x < la; x++)
+                for (int t = 0; t <= x + 1; t++) {
+                    double v = XPA * e[x][0][t] + (t + 1) * e[x][0][t + 1];
+                    if (t > 0) v += e[x][0][t - 1] / (2.0 * p);
+                    e[x + 1][0][t] = v;
+                }
+            for (int y = 0; y < lb; y++)
+                for (int x = 0; x <= la; x++)
+                    for (int t = 0; t <= x + y + 1; t++) {
+                        double v = XPB * e[x][y][t] + (t + 1) * e[x][y][t + 1];
+                        if (t > 0) v += e[x][y][t - 1] / (2.0 * p);
+                        e[x][y + 1][t] = v;
+                    }
+            pp[1 + d] = P;
+        }
+        if (lane == 3) {
+            pp[0] = p;
+            pp[4] = b.coef[b.poff[A] + i] * b.coef[b.poff[B] + j] * exp(-(a * bb / p) * AB2) / p;
+        }
+        __syncthreads();
+        const double scale = pp[4];
+        double lmax = 0.0;
+        for (int idx = lane; idx < esz; idx += 64) {
+            const int ab = idx / H, hc = hcode[idx - ab * H];
+            const int t = hc & 15, u = (hc >> 4) & 15, v = hc >> 8;
+            double sum = 0.0;
+            for (int n = 0; n < nterm[ab]; n++) {
+                const int c = tcode[ab][n];
+                sum += tcoef[ab][n] * e1[0][c & 3][(c >> 6) & 3][t] * e1[1][(c >> 2) & 3][(c >> 8) & 3][u] *
+                       e1[2][(c >> 4) & 3][(c >> 10) & 3][v];
+            }
+            const double val = scale * sum;
+            ent[idx] = val;
+            lmax = fmax(lmax, fabs(val));
+        }
+        for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
+        if (lmax >= prim_eps) {  // the same on every lane
+            double* Eo = E + eoff[pid] + (long)kept * esz;
+            double* Zo = Ez + zoff[pid] + (long)kept * ez;
+            for (int idx = lane; idx < esz; idx += 64) {
+                Eo[idx] = ent[idx];
+                if (zpos[idx] >= 0) Zo[zpos[idx]] = ent[idx];
+            }
+            if (lane == 0) prim[poff[pid] + kept] = make_double4(pp[0], pp[1], pp[2], pp[3]);
+            kept++;
+        }
+        __syncthreads();  // e1 and pp are rewritten for the next primitive pair
+    }
+    if (lane == 0) nprim_out[pid] = kept;
 }
 
 // Row k = (bra pair r, ket segment s).  exact[k] = number of kets j of the
@@ -201,8 +228,7 @@ int gtf_build_pairs(hipStream_t st, const BasisView& b, int npairs, const int* s
                     int* nprim_out, double prim_eps)
 {
     if (npairs <= 0) return 0;
-    build_pairs_kernel<<<(npairs + 3) / 4, 256, 0, st>>>(b, npairs, s1, s2, poff, eoff, prim, E, zoff, Ez, nprim_out,
-                                                       prim_eps);
+    build_pairs_kernel<<<npairs, 64, 0, st>>>(b, npairs, s1, s2, poff, eoff, prim, E, zoff, Ez, nprim_out, prim_eps);
     GTF_CHECK(hipGetLastError());
     return 0;
 }

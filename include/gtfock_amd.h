@@ -18,7 +18,7 @@ extern "C" {
 
 /* out[8] = {usq, uitl, nnz, max Schwarz value, GPU seconds of the last
  *           PFock_computeFock, wall seconds of it, PFock_create seconds,
- *           device bytes held} */
+ *           most device bytes held at once} */
 int gtf_pfock_stats(PFock_t pfock, double *out);
 
 /* Densities the last PFock_computeFock digested from each integral in one
@@ -83,7 +83,7 @@ int gtf_pfock_class_info(PFock_t pfock, int cls, int *info);
 /* The engine's HIP stream (hipStream_t). */
 void *gtf_pfock_stream(PFock_t pfock);
 
-/* Device memory held by the engine, in bytes. */
+/* The most device memory the engine has held at once, in bytes. */
 double gtf_pfock_device_bytes(PFock_t pfock);
 
 /* Tile autotuner.  The tile launches of a quartet class have a shape
@@ -196,7 +196,8 @@ int gtf_batch_device_ptrs(gtf_batch_t batch, double **D, double **G);
 void *gtf_batch_stream(gtf_batch_t batch);
 /* out[11] = {quartets, primitive quartets, FP32 quartets, FP32 primitive
  *            quartets, GPU ms of the last build, graph captured (0/1),
- *            device bytes, create seconds, SCF seconds in Fock builds,
+ *            most device bytes held at once (setup temporaries
+ *            included), create seconds, SCF seconds in Fock builds,
  *            in DIIS, in purification}; returns 11 */
 int gtf_batch_stats(gtf_batch_t batch, double *out);
 /* per class (21): quartets, primitive quartets, and their FP32 parts */

@@ -58,3 +58,60 @@ def c10h22_gold():
     for r in gold.get("reruns", {}).values():
         spread = np.maximum(spread, np.abs(np.array(r) - e))
     return gold, e, np.maximum(1e-9, 10.0 * spread)
+
+
+class HipRuntime:
+    """The HIP runtime the library itself runs on (found in this process's
+    maps after the library is loaded), for device buffers and memory
+    queries in the tests: the one runtime every GPU object of the test
+    shares."""
+
+    def __init__(self):
+        import ctypes
+        import gtfock_amd
+        gtfock_amd.lib()
+        path = None
+        with open("/proc/self/maps") as fh:
+            for line in fh:
+                if "libamdhip64.so" in line:
+                    path = line.split()[-1]
+                    break
+        assert path, "libamdhip64 is not loaded"
+        self.path = path
+        self.rt = ctypes.CDLL(path)
+        self.rt.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.rt.hipFree.argtypes = [ctypes.c_void_p]
+        self.rt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.rt.hipMemGetInfo.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+        self._c = ctypes
+
+    def _ok(self, st, what):
+        assert st == 0, "%s failed (%d)" % (what, st)
+
+    def malloc(self, nbytes):
+        p = self._c.c_void_p()
+        self._ok(self.rt.hipMalloc(self._c.byref(p), nbytes), "hipMalloc")
+        return p.value
+
+    def free(self, p):
+        self._ok(self.rt.hipFree(p), "hipFree")
+
+    def upload(self, p, a):
+        """host array -> device (a contiguous numpy array)"""
+        self._ok(self.rt.hipMemcpy(p, a.ctypes.data, a.nbytes, 1), "hipMemcpy H2D")
+
+    def download(self, a, p):
+        self._ok(self.rt.hipMemcpy(a.ctypes.data, p, a.nbytes, 2), "hipMemcpy D2H")
+
+    def sync(self):
+        self._ok(self.rt.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+    def free_bytes(self):
+        free, total = self._c.c_size_t(), self._c.c_size_t()
+        self._ok(self.rt.hipMemGetInfo(self._c.byref(free), self._c.byref(total)), "hipMemGetInfo")
+        return free.value
+
+
+@pytest.fixture(scope="session")
+def hip():
+    return HipRuntime()

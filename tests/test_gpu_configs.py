@@ -93,7 +93,9 @@ def test_c54h18_mixed_scf_against_fp64(g, gbs, mol_dir):
     tail = [k for k in range(1, n) if abs(e64[k] - e64[k - 1]) < 1e-6]
     assert len(tail) >= 2
     assert max(abs(em[k] - e64[k]) for k in tail) < 1e-6
-    assert min(st["purif_low_iters"]) > 0
+    # split products until the DIIS residual drops below 1e-3, FP64 after
+    low = st["purif_low_iters"]
+    assert low[0] > 0 and low[-1] == 0
     b = g.BasisSet(gbs, xyz)
     pf = g.PFock(b)
     S = pf.overlap()
@@ -207,7 +209,10 @@ def test_serving_stream_with_flakes_to_c96h24(g, guess_gbs, tmp_path):
     lat = np.array([r["t_done"] - r["t_submit"] for r in mix])
     print("served 16 jobs: p50 %.2f s, p99 %.2f s, makespan %.1f s, joins %d, regroups %d" % (
         np.percentile(lat, 50), np.percentile(lat, 99), max(r["t_done"] for r in mix) - t0, joins, regroups))
-    assert all(r["status"] == 2 for r in mix), [(n, r["status"], r["iters"]) for n, r in zip(names, mix)]
+    for n, r in zip(names, mix):
+        print("  %-8s status %d, %d builds, cohort %d, E %.10f, latency %.2f s" % (
+            n, r["status"], r["iters"], r["cohort"], r["energy"], r["t_done"] - r["t_submit"]))
+    assert all(r["status"] == 2 for r in mix)
     # FP64 reference energies of the same molecules, one FP64 batch
     ref = g.Batch(bases)
     e64, it64 = ref.scf(max_iters=40, ediff=1e-8)

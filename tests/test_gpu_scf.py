@@ -102,7 +102,7 @@ def test_scf_water_matches_oracle(g, gbs, mol_dir):
 
 
 @gpu
-@pytest.mark.parametrize("n", [16, 64, 100, 250])
+@pytest.mark.parametrize("n", [16, 64, 100, 250, 846, 1464])
 def test_mfma_dgemm(g, n):
     rng = np.random.default_rng(n)
     A = rng.normal(size=(n, n))
