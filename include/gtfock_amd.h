@@ -129,6 +129,18 @@ int gtf_gemm_host(int n, const double *A, const double *B, double *C, int transB
  * row maxima. */
 int gtf_gemm_split_host(int n, const double *A, const double *B, double *C, int transB, int ns);
 
+/* gtf_gemm_split_host on caller-owned device matrices with no allocation.
+ * No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * A, B, C: row-major ld x ld device arrays, zero beyond n (ld a multiple of
+ * 64, at most 1536); stream: a hipStream_t (NULL: the default stream); the
+ * call is asynchronous on it.  The scratch is one static device array per
+ * GPU (55 MiB, held until the process ends), shared by every call on that
+ * GPU: calls that overlap on one GPU race.  Returns 0, -3 (invalid or too
+ * large) or -1 (HIP failure). */
+int gtf_gemm_split_noalloc(int n, int ld, const double *A, const double *B, double *C, int transB, int ns,
+                           void *stream);
+
 /* Mixed-precision purification (compute_purification): mode 1 uses FP32
  * MFMA for the D^2 and D^3 products, mode 2 the split-BF16 product (FP64
  * residuals carried slice to slice), while ||D - D^2||_F >= until, and FP64
