@@ -403,18 +403,20 @@ def serve_leg(g, gbs, njobs, rate, seed, tau, max_rings, budget_gb, cohorts, max
            "p99_s": float(np.percentile(lat, 99)), "mean_builds": float(np.mean([r["iters"] for r in ok])),
            "not_converged": int(sum(1 for r in ok if r["iters"] >= max_iters or r["iters"] < 0))}
     if ref:
-        # one FP64 SCF per distinct molecule (the stream repeats molecules)
+        # one FP64 SCF per distinct geometry (the stream repeats molecules;
+        # water clusters of one size differ in their random placement)
+        geo = [geometry.to_xyz(atoms) for _, atoms in mols]
         first = {}
-        for k, (name, _) in enumerate(mols):
-            first.setdefault(name, k)
+        for k, x in enumerate(geo):
+            first.setdefault(x, k)
         keys = sorted(first.values())
         log("serve: FP64 reference batch of the %d distinct molecules" % len(keys))
         bt = g.Batch([bases[k] for k in keys])
         e64u, it64u = scf_logged(bt, max_iters, 1e-8, "serve FP64 reference")
         bt.close()
-        ref64 = {mols[k][0]: (e64u[i], it64u[i]) for i, k in enumerate(keys)}
-        e64 = [ref64[name][0] for name, _ in mols]
-        it64 = [ref64[name][1] for name, _ in mols]
+        ref64 = {geo[k]: (e64u[i], it64u[i]) for i, k in enumerate(keys)}
+        e64 = [ref64[x][0] for x in geo]
+        it64 = [ref64[x][1] for x in geo]
         errs = [abs(r["energy"] - e64[k]) for k, r in enumerate(res)
                 if r["status"] == 2 and 0 < r["iters"] < max_iters and 0 < it64[k] < max_iters]
         out["max_abs_dE_vs_fp64_Eh"] = float(max(errs)) if errs else None

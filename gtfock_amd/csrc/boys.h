@@ -23,6 +23,10 @@
 template <int M>
 __device__ __forceinline__ void boys_eval8(double T, double* F, const double* __restrict__ slab)
 {
+    // exp(-T) serves both regimes; computed once before the branch, so a
+    // wavefront with lanes on both sides does not run it twice
+    double e = 0.0;
+    if constexpr (M > 0) e = exp(-T);
     if (T < BOYS_TMAX) {
         const int i = (int)(T * BOYS8_INV_DELTA + 0.5);
         const double dt = i * BOYS8_DELTA - T;
@@ -37,7 +41,6 @@ __device__ __forceinline__ void boys_eval8(double T, double* F, const double* __
         f = fma(f, dt, c01.x);
         F[M] = f;
         if constexpr (M > 0) {
-            const double e = exp(-T);
             const double t2 = T + T;
             static_for<0, M>([&](auto k) {
                 constexpr int m = M - 1 - decltype(k)::value;
@@ -48,7 +51,6 @@ __device__ __forceinline__ void boys_eval8(double T, double* F, const double* __
         const double rt = rsqrt_nr(T);
         F[0] = 0.88622692545275801365 * rt;  // sqrt(pi)/2
         if constexpr (M > 0) {
-            const double e = exp(-T);
             const double h = 0.5 * rt * rt;
             static_for<1, M + 1>([&](auto k) {
                 constexpr int m = decltype(k)::value;
