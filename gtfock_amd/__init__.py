@@ -15,6 +15,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libgtfock_hip.so")
 DATA_DIR = os.path.join(_HERE, "data")
+# set-up phases timed by PFock.setup_times / Batch.setup_times
+SETUP_PHASES = ("basis", "pairs", "schwarz", "sig_order", "row_count", "tiles")
 # fields of a quartet class's tile shape (gtf_pfock_tile_shape)
 from .tuning import SHAPE_KEYS, flat as _flat_shapes  # noqa: E402
 CSRC_DIR = os.path.join(_HERE, "csrc")
@@ -106,6 +108,7 @@ def lib():
         "gtf_pfock_stats": ([_vp, _dp], ctypes.c_int),
         "gtf_pfock_fused_densities": ([_vp], ctypes.c_int),
         "gtf_pfock_balance": ([_vp, _dp], ctypes.c_int),
+        "gtf_pfock_setup_times": ([_vp, _dp], ctypes.c_int),
         "gtf_pfock_csr": ([_vp, _ip, _ip, _ip, _dp], ctypes.c_int),
         "gtf_pfock_class_counts": ([_vp, _lp], ctypes.c_int),
         "gtf_pfock_quartets": ([_vp, _ip, ctypes.c_long], ctypes.c_long),
@@ -169,6 +172,7 @@ def lib():
         "gtf_batch_device_ptrs": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], ctypes.c_int),
         "gtf_batch_stream": ([_vp], _vp),
         "gtf_batch_stats": ([_vp, _dp], ctypes.c_int),
+        "gtf_batch_setup_times": ([_vp, _dp], ctypes.c_int),
         "gtf_batch_class_work": ([_vp, _lp, _dp, _lp, _dp], ctypes.c_int),
         "gtf_batch_scf": ([_vp, ctypes.c_int, ctypes.c_double, _dp, _ip, _dp], ctypes.c_int),
         "gtf_batch_done_times": ([_vp, _dp], ctypes.c_int),
@@ -325,6 +329,13 @@ class PFock:
         out = np.zeros(3)
         _check(lib().gtf_pfock_balance(self.h, _p(out)), "gtf_pfock_balance")
         return dict(load_balance=out[0], steals=int(out[1]), workgroups=int(out[2]))
+
+    def setup_times(self):
+        """Wall seconds of PFock_create's set-up phases."""
+        out = np.zeros(6)
+        if lib().gtf_pfock_setup_times(self.h, _p(out)) != 6:
+            raise GtfError("gtf_pfock_setup_times failed")
+        return dict(zip(SETUP_PHASES, out.tolist()))
 
     def csr(self):
         ns = self.basis.nshells
@@ -672,6 +683,13 @@ class Batch:
         keys = ["nquartets", "nprimquartets", "nq_fp32", "npq_fp32", "gpu_ms", "graph", "device_bytes",
                 "create_s", "scf_fock_s", "scf_diis_s", "scf_purif_s"]
         return dict(zip(keys, out.tolist()))
+
+    def setup_times(self):
+        """Wall seconds of the set-up phases: creation (as PFock's), then the
+        SCF arrays with the one-electron matrices, and X = S^-1/2."""
+        out = np.zeros(8)
+        self._st(0 if lib().gtf_batch_setup_times(self.h, _p(out)) == 8 else -3, "gtf_batch_setup_times")
+        return dict(zip(SETUP_PHASES + ("scf_oneel", "scf_xmat"), out.tolist()))
 
     def class_work(self):
         nq = np.zeros(21, dtype=np.int64)

@@ -63,6 +63,8 @@ struct GtfBatch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     double last_ms = 0.0, create_s = 0.0;
+    // SCF set-up: one-electron matrices (and the SCF arrays), X = S^-1/2
+    double setup_oneel_s = 0.0, setup_xmat_s = 0.0;
     long nbuilds = 0;
     // SCF state
     bool scf_ready = false;
@@ -248,6 +250,7 @@ int GtfBatch::scf_setup(const GtfBatch* from, const int* from_idx)
     if (scf_ready) return 0;
     auto carried = [&](int m) { return from && from_idx && from_idx[m] >= 0; };
     GtfSystem& s = sys;
+    const double t0 = bwtime();
     d_H = s.dalloc<double>(total);
     d_S = s.dalloc<double>(total);
     d_F = s.dalloc<double>(total);
@@ -279,15 +282,24 @@ int GtfBatch::scf_setup(const GtfBatch* from, const int* from_idx)
         if (gtf_fill(stream, *p, ptotal, 0.0)) return -1;
     if (gtf_fill(stream, evecs, ptotal * B_MAX_DIIS, 0.0) || gtf_fill(stream, Fvecs, ptotal * B_MAX_DIIS, 0.0))
         return -1;
-    // one-electron matrices per molecule over its slice of the shell arrays
+    // one-electron matrices of the molecules not carried over, in one
+    // launch: each over its slice of the shell and atom arrays
+    std::vector<OneelMol> om;
+    int nblk = 0;
     for (int m = 0; m < nmol; m++) {
         if (carried(m)) continue;
-        const int s0 = s.mol_shell_off[m], a0 = atom_off[m];
-        BasisView bv{s.mol_nshells[m], n[m], bases[m]->natoms, s.d_L + s0, s.d_nprim + s0, s.d_poff + s0,
-                     s.d_fstart + s0, s.d_atom + s0, s.d_alpha, s.d_coef, s.d_center + 3 * s0,
-                     s.d_atom_xyz + 3 * a0, s.d_atom_Z + a0};
-        if (gtf_oneel(stream, bv, s.d_boys, n[m], d_S + off[m], T + off[m], V + off[m])) return -1;
+        const int ns = s.mol_nshells[m];
+        om.push_back(OneelMol{nblk, s.mol_shell_off[m], ns, atom_off[m], bases[m]->natoms, n[m], off[m]});
+        nblk += (ns * ns + 63) / 64;
     }
+    const int nom = (int)om.size();
+    om.push_back(OneelMol{nblk, 0, 0, 0, 0, 0, 0});
+    OneelMol* d_om = s.dalloc<OneelMol>(om.size());
+    if (!d_om) return -2;
+    B_CHECK(hipMemcpyAsync(d_om, om.data(), sizeof(OneelMol) * om.size(), hipMemcpyHostToDevice, stream));
+    const BasisView ball{s.nshells, s.nbf, s.natoms, s.d_L, s.d_nprim, s.d_poff, s.d_fstart, s.d_atom,
+                         s.d_alpha, s.d_coef, s.d_center, s.d_atom_xyz, s.d_atom_Z};
+    if (gtf_oneel_batch(stream, ball, s.d_boys, d_om, nom, nblk, d_S, T, V)) return -1;
     if (gtf_add(stream, T, V, d_H, total)) return -1;
     for (int m = 0; m < nmol; m++) {
         if (!carried(m)) continue;
@@ -316,6 +328,9 @@ int GtfBatch::scf_setup(const GtfBatch* from, const int* from_idx)
             prev = order[k];
         }
     }
+    B_CHECK(hipStreamSynchronize(stream));  // om is read by the copy above
+    s.dfree(d_om);
+    const double t1 = bwtime();
     size_t wmax = 1;
     for (const auto& gr : groups) wmax = std::max(wmax, gr.size() * (size_t)n[gr[0]] * n[gr[0]]);
     double* W = s.dalloc<double>(wmax);
@@ -372,6 +387,8 @@ int GtfBatch::scf_setup(const GtfBatch* from, const int* from_idx)
     s.dfree(lam);
     s.dfree(work);
     s.dfree(info);
+    setup_oneel_s = t1 - t0;
+    setup_xmat_s = bwtime() - t1;
     scf_ready = true;
     return 0;
 }
@@ -889,6 +906,15 @@ int gtf_batch_stats(gtf_batch_t b, double* out)
     return 11;
 }
 
+int gtf_batch_setup_times(gtf_batch_t b, double* out)
+{
+    if (!b || !out) return -3;
+    for (int k = 0; k < GtfSystem::NSETUP; k++) out[k] = b->sys.setup_s[k];
+    out[GtfSystem::NSETUP] = b->setup_oneel_s;
+    out[GtfSystem::NSETUP + 1] = b->setup_xmat_s;
+    return GtfSystem::NSETUP + 2;
+}
+
 int gtf_batch_class_work(gtf_batch_t b, long* nquartets, double* nprimquartets, long* nq_fp32, double* npq_fp32)
 {
     if (!b) return -3;
@@ -1048,7 +1074,7 @@ double gtf_batch_job_bytes(BasisSet_t basis)
             int la = b->L[s], lb = b->L[t];
             if (lb > la) std::swap(la, lb);
             const double np = (double)b->nprim[s] * b->nprim[t];
-            pairs += np * (32.0 + 8.0 * (nsph(la) * nsph(lb) * nherm(la + lb) + herm_nnz(la, lb))) + 96.0;
+            pairs += np * (32.0 + 8.0 * (nsph(la) * nsph(lb) * nherm(la + lb) + herm_nnz_lookup(la, lb))) + 96.0;
         }
     const double n = b->nbf, pl = ((b->nbf + 63) / 64) * 64;
     // + the split purification's BF16 slices of both operands at the most

@@ -2,13 +2,24 @@
 // quartet compaction and the per-class ERI launches.
 #include "engine.h"
 
+#include <sys/time.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <atomic>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "screen.h"
+
+static double sys_wtime()
+{
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
 
 void gtf_class_types(int cls, int& A, int& B)
 {
@@ -40,6 +51,7 @@ int GtfSystem::init(const std::vector<const BasisSet*>& mols, hipStream_t st)
     stream = st;
     nmol = (int)mols.size();
     if (nmol == 0) return -3;
+    double t0 = sys_wtime();
     nbf = nshells = natoms = 0;
     mol_nbf.assign(nmol, 0);
     mol_shell_off.assign(nmol, 0);
@@ -122,6 +134,8 @@ int GtfSystem::init(const std::vector<const BasisSet*>& mols, hipStream_t st)
     if (upload(d_boys8, b8, st)) return -1;
     // the uploads read host vectors that go out of scope here
     SYS_CHECK(hipStreamSynchronize(st));
+    setup_s[0] = sys_wtime() - t0;
+    t0 = sys_wtime();
 
     // all unordered shell pairs (A >= B by local index) of every molecule,
     // canonical orientation by L
@@ -151,7 +165,7 @@ int GtfSystem::init(const std::vector<const BasisSet*>& mols, hipStream_t st)
                 np += h_pnprim[id];
                 ne += (long)h_pnprim[id] * nsph(la) * nsph(lb) * nherm(la + lb);
                 pzoff[id] = nz;
-                nz += (long)h_pnprim[id] * herm_nnz(la, lb);
+                nz += (long)h_pnprim[id] * herm_nnz_lookup(la, lb);
             }
     }
     nprimpairs = np;
@@ -180,6 +194,7 @@ int GtfSystem::init(const std::vector<const BasisSet*>& mols, hipStream_t st)
     SYS_CHECK(hipStreamSynchronize(st));
     nprimpairs_kept = 0;
     for (int id = 0; id < npairs; id++) nprimpairs_kept += h_pnprim[id];
+    setup_s[1] = sys_wtime() - t0;
     return 0;
 }
 
@@ -227,6 +242,7 @@ int GtfSystem::screen(double tolscr)
     }
     tol = tolscr;
     tol2 = tolscr * tolscr;
+    double t0 = sys_wtime();
     const PairView pv = pair_view();
     unsigned long long* d_vbits = dalloc<unsigned long long>(npairs);
     int2* d_q = dalloc<int2>(npairs);
@@ -269,6 +285,8 @@ int GtfSystem::screen(double tolscr)
         mol_vmax[h_pair_mol[id]] = std::max(mol_vmax[h_pair_mol[id]], v);
     }
     vmax = mol_vmax[0];
+    setup_s[2] = sys_wtime() - t0;
+    t0 = sys_wtime();
     // significant pairs of molecule m: v > tol^2 / vmax_m (pfock/screening.c:111,129),
     // in one segment per (molecule, primitive-count bucket), compacted and
     // ordered on the device (sigsort.hip); the host keeps copies for the
@@ -364,6 +382,8 @@ int GtfSystem::screen(double tolscr)
     dfree(d_bcount);
     dfree(d_boff);
     dfree(d_total);
+    setup_s[3] = sys_wtime() - t0;
+    t0 = sys_wtime();
     // quartet rows per class: bra type A >= ket type B; a row pairs one bra
     // with one non-empty ket segment of its molecule
     int maxrows = 1;
@@ -429,6 +449,7 @@ int GtfSystem::screen(double tolscr)
     }
     dfree(d_counts);
     dfree(d_exact);
+    setup_s[4] = sys_wtime() - t0;
     if (build_tiles()) return -1;
     screened = true;
     return 0;
@@ -517,9 +538,64 @@ void GtfSystem::csr(std::vector<int>& shellptr, std::vector<int>& shellid, std::
 
 int GtfSystem::build_tiles()
 {
-    for (int cls = 0; cls < GTF_NCLASSES; cls++)
-        if (build_class_tiles(cls)) return -1;
+    const double t0 = sys_wtime();
+    // the previous tiles may still be read by a launch in flight
     SYS_CHECK(hipStreamSynchronize(stream));
+    // fill the shared prefix and segment-maximum caches first: the classes
+    // are then cut on host threads that only read shared state
+    for (int t = 0; t < GTF_NPAIRTYPES; t++) {
+        list_prim_prefix(t);
+        seg_nprim_max(t, 0);
+    }
+    std::vector<int4> hv[GTF_NCLASSES], lv[GTF_NCLASSES];
+    {
+        const int nth = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
+        std::atomic<int> next{0};
+        auto work = [&]() {
+            for (int c; (c = next.fetch_add(1)) < GTF_NCLASSES;) cut_class_tiles(c, hv[c], lv[c]);
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nth; t++) pool.emplace_back(work);
+        work();
+        for (std::thread& t : pool) t.join();
+    }
+    for (int cls = 0; cls < GTF_NCLASSES; cls++)
+        if (upload_class_tiles(cls, hv[cls], lv[cls])) return -1;
+    // the uploads read hv / lv
+    SYS_CHECK(hipStreamSynchronize(stream));
+    setup_s[5] = sys_wtime() - t0;
+    return 0;
+}
+
+int GtfSystem::build_class_tiles(int cls)
+{
+    int A, B;
+    gtf_class_types(cls, A, B);
+    SYS_CHECK(hipStreamSynchronize(stream));
+    list_prim_prefix(B);
+    seg_nprim_max(B, 0);
+    std::vector<int4> hv, lv;
+    cut_class_tiles(cls, hv, lv);
+    if (upload_class_tiles(cls, hv, lv)) return -1;
+    SYS_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+// replaces the device tiles of a class by hv (FP64) and lv (FP32); the
+// stream must have no launch in flight that reads the old ones
+int GtfSystem::upload_class_tiles(int cls, const std::vector<int4>& hv, const std::vector<int4>& lv)
+{
+    dfree(d_tiles[cls]);
+    dfree(d_tiles_lo[cls]);
+    d_tiles[cls] = d_tiles_lo[cls] = nullptr;
+    if (!hv.empty()) {
+        d_tiles[cls] = dalloc<int4>(hv.size());
+        if (!d_tiles[cls] || upload(d_tiles[cls], hv, stream)) return -1;
+    }
+    if (!lv.empty()) {
+        d_tiles_lo[cls] = dalloc<int4>(lv.size());
+        if (!d_tiles_lo[cls] || upload(d_tiles_lo[cls], lv, stream)) return -1;
+    }
     return 0;
 }
 
@@ -528,8 +604,9 @@ int GtfSystem::build_tiles()
 // tau > 0 each row splits at the first ket whose Schwarz bound
 // sqrt(v_bra v_ket) < tau: FP64 tiles cover the block-padded prefix, FP32
 // tiles start at the block holding the split; the kernels re-test the exact
-// predicates per quartet.
-int GtfSystem::build_class_tiles(int cls)
+// predicates per quartet.  Host work only; reads the class's own fields and
+// shared state that list_prim_prefix / seg_nprim_max have cached already.
+void GtfSystem::cut_class_tiles(int cls, std::vector<int4>& hv, std::vector<int4>& lv)
 {
     const double tau2 = class_tau2(cls);
     {
@@ -550,12 +627,9 @@ int GtfSystem::build_class_tiles(int cls)
         ntiles[cls] = ntiles_lo[cls] = 0;
         nq_lo[cls] = 0;
         nprimq_lo[cls] = 0.0;
-        // the previous tiles may still be read by a launch in flight
-        SYS_CHECK(hipStreamSynchronize(stream));
-        dfree(d_tiles[cls]);
-        dfree(d_tiles_lo[cls]);
-        d_tiles[cls] = d_tiles_lo[cls] = nullptr;
-        if (nq[cls] == 0 || !use_tiles || tile_lds[cls] > max_lds) return 0;
+        hv.clear();
+        lv.clear();
+        if (nq[cls] == 0 || !use_tiles || tile_lds[cls] > max_lds) return;
         const std::vector<double>& vs = h_sorted[B];
         const std::vector<double> kp = ket_prim_prefix(B);
         // Tiles of the kets [b, e) of one row of segment sg, cut by work units
@@ -565,7 +639,7 @@ int GtfSystem::build_class_tiles(int cls)
         // count grows by a constant per ket, and its primitive-quartet cost is
         // read off the prefix sums of the list, so each tile end is one binary
         // search.
-        const std::vector<double>& lp = list_prim_prefix(B);
+        const std::vector<double>& lp = h_listprim[B];
         const bool exch = exchanged(cls);
         std::vector<std::pair<double, int4>> hi, lo;
         auto cut = [&](std::vector<std::pair<double, int4>>& out, int r, int nb, int nkmax, long b, long e) {
@@ -609,7 +683,7 @@ int GtfSystem::build_class_tiles(int cls)
             }
             const long pad = std::min<long>(nket, ((c + GTF_KET_BLOCK - 1) / GTF_KET_BLOCK) * GTF_KET_BLOCK);
             const long pad64 = std::min<long>(nket, ((c64 + GTF_KET_BLOCK - 1) / GTF_KET_BLOCK) * GTF_KET_BLOCK);
-            const int nkmax = seg_nprim_max(B, sg);
+            const int nkmax = h_segmax[B][sg];
             cut(hi, r, nb, nkmax, s0, s0 + pad64);
             if (c > c64) {
                 const long b0 = (c64 / GTF_KET_BLOCK) * GTF_KET_BLOCK;
@@ -624,22 +698,15 @@ int GtfSystem::build_class_tiles(int cls)
         };
         std::stable_sort(hi.begin(), hi.end(), heavy_first);
         std::stable_sort(lo.begin(), lo.end(), heavy_first);
-        std::vector<int4> hv, lv;
+        hv.reserve(hi.size());
+        lv.reserve(lo.size());
         for (auto& x : hi) hv.push_back(x.second);
         for (auto& x : lo) lv.push_back(x.second);
         ntiles[cls] = (int)hv.size();
         ntiles_lo[cls] = (int)lv.size();
-        if (!hv.empty()) {
-            d_tiles[cls] = dalloc<int4>(hv.size());
-            if (!d_tiles[cls] || upload(d_tiles[cls], hv, stream)) return -1;
-        }
-        if (!lv.empty()) {
-            d_tiles_lo[cls] = dalloc<int4>(lv.size());
-            if (!d_tiles_lo[cls] || upload(d_tiles_lo[cls], lv, stream)) return -1;
-        }
     }
-    return 0;
 }
+
 
 bool GtfSystem::fits_densities(int nd) const
 {

@@ -166,6 +166,11 @@ struct GtfSystem {
     {
         return balance && d_wgstat && grid == wg_n[k] ? d_wgstat + 3 * wg_off[k] : nullptr;
     }
+    // wall seconds of the set-up phases, each ending with the stream
+    // synchronised: basis upload, primitive pairs, Schwarz diagonal,
+    // significant-pair ordering, quartet-row counting, tile cutting
+    static constexpr int NSETUP = 6;
+    double setup_s[NSETUP] = {0};
     // device bytes the system holds now, and the most it held at once
     size_t live_bytes = 0, dev_bytes = 0;
     std::vector<void*> allocs;
@@ -175,8 +180,12 @@ struct GtfSystem {
     int init(const BasisSet* b, hipStream_t st) { return init(std::vector<const BasisSet*>{b}, st); }
     // Schwarz screening (pfock/screening.c:15-180) and quartet compaction
     int screen(double tolscr);
+    // tiles of every class (host cutting on threads, then the uploads), or
+    // of one class
     int build_tiles();
     int build_class_tiles(int cls);
+    void cut_class_tiles(int cls, std::vector<int4>& hv, std::vector<int4>& lv);
+    int upload_class_tiles(int cls, const std::vector<int4>& hv, const std::vector<int4>& lv);
     // CSR of molecule 0 in the reference layout (host)
     void csr(std::vector<int>& shellptr, std::vector<int>& shellid, std::vector<int>& shellrid,
              std::vector<double>& shellvalue) const;

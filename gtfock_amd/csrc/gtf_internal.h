@@ -212,6 +212,16 @@ __host__ __device__ constexpr int herm_nnz(int la, int lb)
             if (herm_nz(la, lb, ab / nsph(lb), ab % nsph(lb), h)) n++;
     return n;
 }
+// herm_nnz read from the compile-time tables (the runtime form takes
+// thousands of steps per call; set-up code calls it once per shell pair)
+inline int herm_nnz_lookup(int la, int lb)
+{
+    static_assert(GTF_LMAX == 2, "table below lists l = 0..2");
+    static constexpr int t[3][3] = {{NzTab<0, 0>::data.nnz, NzTab<0, 1>::data.nnz, NzTab<0, 2>::data.nnz},
+                                    {NzTab<1, 0>::data.nnz, NzTab<1, 1>::data.nnz, NzTab<1, 2>::data.nnz},
+                                    {NzTab<2, 0>::data.nnz, NzTab<2, 1>::data.nnz, NzTab<2, 2>::data.nnz}};
+    return t[la][lb];
+}
 __host__ __device__ constexpr int herm_nz_pos(int la, int lb, int ab, int h)
 {
     int n = 0;

@@ -10,10 +10,11 @@
 
 #define LKIN (GTF_LMAX + 2)
 
-__global__ void oneel_kernel(BasisView b, const double* __restrict__ boys, int ld, double* __restrict__ S,
-                             double* __restrict__ Tk, double* __restrict__ V)
+// S, T, V of ordered shell pair pid = A * nshells + B of one molecule
+__device__ __forceinline__ void oneel_pair(const BasisView& b, const double* __restrict__ boys, int ld,
+                                           double* __restrict__ S, double* __restrict__ Tk, double* __restrict__ V,
+                                           int pid)
 {
-    const int pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= b.nshells * b.nshells) return;
     const int A = pid / b.nshells, B = pid % b.nshells;
     const int la = b.L[A], lb = b.L[B], lab = la + lb;
@@ -133,6 +134,47 @@ __global__ void oneel_kernel(BasisView b, const double* __restrict__ boys, int l
             Tk[k] = t;
             V[k] = v;
         }
+}
+
+__global__ void oneel_kernel(BasisView b, const double* __restrict__ boys, int ld, double* __restrict__ S,
+                             double* __restrict__ Tk, double* __restrict__ V)
+{
+    oneel_pair(b, boys, ld, S, Tk, V, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Many molecules in one launch: molecule k owns the 64-thread blocks
+// [mol[k].blk0, mol[k + 1].blk0) (mol[nmol].blk0 = the grid), one thread per
+// ordered shell pair, and views the shared basis arrays from its own shell
+// and atom offsets.  Each thread computes exactly what oneel_kernel does
+// for that molecule alone.
+__global__ __launch_bounds__(64) void oneel_batch_kernel(BasisView b, const double* __restrict__ boys,
+                                                         const OneelMol* __restrict__ mol, int nmol,
+                                                         double* __restrict__ S, double* __restrict__ Tk,
+                                                         double* __restrict__ V)
+{
+    const int blk = blockIdx.x;
+    int lo = 0, hi = nmol - 1;  // last k with mol[k].blk0 <= blk
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (mol[mid].blk0 <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    const OneelMol m = mol[lo];
+    const BasisView v{m.nshells,           b.nbf,          m.natoms,
+                      b.L + m.shell_off,   b.nprim + m.shell_off, b.poff + m.shell_off,
+                      b.fstart + m.shell_off, b.atom + m.shell_off, b.alpha,
+                      b.coef,              b.center + 3 * m.shell_off, b.atom_xyz + 3 * m.atom_off,
+                      b.atom_Z + m.atom_off};
+    oneel_pair(v, boys, m.ld, S + m.moff, Tk + m.moff, V + m.moff, (blk - m.blk0) * 64 + threadIdx.x);
+}
+
+int gtf_oneel_batch(hipStream_t st, const BasisView& b, const double* boys, const OneelMol* d_mol, int nmol,
+                    int nblocks, double* S, double* T, double* V)
+{
+    if (nmol <= 0 || nblocks <= 0) return 0;
+    oneel_batch_kernel<<<nblocks, 64, 0, st>>>(b, boys, d_mol, nmol, S, T, V);
+    GTF_CHECK(hipGetLastError());
+    return 0;
 }
 
 int gtf_oneel(hipStream_t st, const BasisView& b, const double* boys, int ld, double* S, double* T, double* V)

@@ -538,6 +538,13 @@ int gtf_pfock_balance(PFock_t pf, double* out)
     return 0;
 }
 
+int gtf_pfock_setup_times(PFock_t pf, double* out)
+{
+    if (!pf || !out) return -1;
+    for (int k = 0; k < GtfSystem::NSETUP; k++) out[k] = pf->sys.setup_s[k];
+    return GtfSystem::NSETUP;
+}
+
 int gtf_pfock_stats(PFock_t pf, double* out)
 {
     out[0] = pf->usq;

@@ -34,6 +34,13 @@ int gtf_pfock_fused_densities(PFock_t pfock);
  * workgroup's first; out[2] = workgroups.  Returns 0, or -1. */
 int gtf_pfock_balance(PFock_t pfock, double *out);
 
+/* Wall seconds of PFock_create's set-up phases (the work of
+ * pfock/pfock.c:801-960, screening at :922): out[0] basis upload, out[1]
+ * primitive shell pairs, out[2] Schwarz diagonal (MN|MN), out[3]
+ * significant-pair ordering, out[4] quartet-row counting, out[5] tile
+ * cutting.  Returns 6, or -1. */
+int gtf_pfock_setup_times(PFock_t pfock, double *out);
+
 /* The screening CSR in the layout of pfock/screening.c:152-175
  * (shellptr[nshells+1], shellid/shellrid/shellvalue[nnz]); returns nnz. */
 int gtf_pfock_csr(PFock_t pfock, int *shellptr, int *shellid, int *shellrid, double *shellvalue);
@@ -212,6 +219,11 @@ void *gtf_batch_stream(gtf_batch_t batch);
  *            included), create seconds, SCF seconds in Fock builds,
  *            in DIIS, in purification}; returns 11 */
 int gtf_batch_stats(gtf_batch_t batch, double *out);
+/* Wall seconds of the batch's set-up phases: out[0..5] as
+ * gtf_pfock_setup_times (gtf_batch_create), out[6] the SCF arrays and
+ * one-electron matrices, out[7] X = S^-1/2 (the first SCF call or
+ * gtf_batch_scf_begin).  Returns 8. */
+int gtf_batch_setup_times(gtf_batch_t batch, double *out);
 /* per class (21): quartets, primitive quartets, and their FP32 parts */
 int gtf_batch_class_work(gtf_batch_t batch, long *nquartets, double *nprimquartets, long *nq_fp32,
                          double *npq_fp32);

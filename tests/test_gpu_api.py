@@ -171,3 +171,26 @@ def test_statistics_report_measured_balance(g, gbs, mol_dir, capfd):
     out = capfd.readouterr().out
     assert "load blance = %.3f" % bal["load_balance"] in out
     pf.close()
+
+
+@gpu
+def test_setup_phase_times(g, gbs, mol_dir):
+    """The set-up phase timers cover PFock_create's and a batch's set-up:
+    every phase measured, their sum within the wall time of the call."""
+    b = g.BasisSet(gbs, str(mol_dir / "c10h22.xyz"))
+    pf = g.PFock(b)
+    ph = pf.setup_times()
+    print(ph)
+    assert tuple(ph) == g.SETUP_PHASES
+    assert all(v >= 0.0 for v in ph.values()) and ph["pairs"] > 0.0 and ph["tiles"] > 0.0
+    assert sum(ph.values()) <= pf.stats()["create_seconds"] + 1e-3
+    pf.close()
+    w = g.BasisSet(gbs, str(mol_dir / "water.xyz"))
+    bt = g.Batch([b, w, b])
+    bt.scf_begin(max_iters=2, ediff=1e-8)
+    bph = bt.setup_times()
+    print(bph)
+    assert len(bph) == len(g.SETUP_PHASES) + 2
+    assert bph["scf_oneel"] > 0.0 and bph["scf_xmat"] > 0.0
+    assert sum(bph[k] for k in g.SETUP_PHASES) <= bt.stats()["create_s"] + 1e-3
+    bt.close()
