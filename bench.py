@@ -564,6 +564,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c10h22")
     ap.add_argument("--tol", type=float, default=1e-11)
+    ap.add_argument("--prim-tol", type=float, default=1e-15,
+                    help="primitive screening threshold of the builds (the library's default; 0: none)")
     ap.add_argument("--tau", type=float, default=1e-4, help="Schwarz-bound threshold of the FP32 ERIs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--classes", action="store_true", help="print the per-class breakdown to stderr")
@@ -632,6 +634,7 @@ def main():
     t0 = time.perf_counter()
     pf = g.PFock(basis, tolscr=args.tol)
     t_create = time.perf_counter() - t0
+    pf.set_prim_tol(args.prim_tol)
     D = first_density(g, pf, basis)
     pf.put_den_mat(D)
     pf.commit()
@@ -661,7 +664,11 @@ def main():
     value = args.steps / (t1 - t0)
 
     ms, launches = pf.class_times()
-    nq, npq = pf.class_work()
+    nq, npq_all = pf.class_work()
+    # the primitive quartets the kernels evaluate at the build's primitive
+    # threshold (the algorithmic FLOPs count executed work only)
+    npq, npq_lo = pf.prim_census()
+    npq = npq + npq_lo
     names = g.class_names()
     rows = []
     for c in range(21):
@@ -686,7 +693,10 @@ def main():
                 "traffic_source": "copy-calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of this kernel "
                                   "on this workload (profiles/eri_traffic_%s.json)" % args.workload,
                 "algorithmic_flops_per_launch": dom_fl, "avg_launch_ms": dom_s * 1e3,
-                "build_tflops": total_fl / (ms_step * 1e-3) / 1e12}
+                "build_tflops": total_fl / (ms_step * 1e-3) / 1e12,
+                "prim_tol": args.prim_tol,
+                "primitive_quartets_evaluated": float(npq.sum()),
+                "primitive_quartets_screened_quartets": float(npq_all.sum())}
 
     # the same kernels each alone on the GPU (class launches serialised), after
     # the timed region: the per-kernel efficiency without sharing the device

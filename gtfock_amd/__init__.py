@@ -120,6 +120,8 @@ def lib():
         "gtf_pfock_set_timing": ([_vp, ctypes.c_int], ctypes.c_int),
         "gtf_pfock_set_serial": ([_vp, ctypes.c_int], ctypes.c_int),
         "gtf_pfock_set_precision": ([_vp, ctypes.c_double], ctypes.c_int),
+        "gtf_pfock_set_prim_tol": ([_vp, ctypes.c_double], ctypes.c_int),
+        "gtf_pfock_prim_census": ([_vp, _dp, _dp], ctypes.c_int),
         "gtf_pfock_mixed_counts": ([_vp, _lp, _dp], ctypes.c_int),
         "gtf_pfock_class_times": ([_vp, _dp, _lp], ctypes.c_int),
         "gtf_pfock_device_bytes": ([_vp], ctypes.c_double),
@@ -164,6 +166,7 @@ def lib():
         "gtf_batch_layout": ([_vp, _ip, _lp], ctypes.c_int),
         "gtf_batch_total": ([_vp], ctypes.c_long),
         "gtf_batch_set_precision": ([_vp, ctypes.c_double], ctypes.c_int),
+        "gtf_batch_set_prim_tol": ([_vp, ctypes.c_double], ctypes.c_int),
         "gtf_batch_use_graph": ([_vp, ctypes.c_int], ctypes.c_int),
         "gtf_batch_put_density": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
         "gtf_batch_compute_fock": ([_vp], ctypes.c_int),
@@ -368,6 +371,17 @@ class PFock:
     def set_precision(self, tau):
         """Mixed precision: FP32 for quartets with Schwarz bound < tau (0: FP64)."""
         _check(lib().gtf_pfock_set_precision(self.h, float(tau)), "gtf_pfock_set_precision")
+
+    def set_prim_tol(self, tol):
+        """Primitive screening: skip primitive quartets with Schwarz bound < tol (0: none)."""
+        _check(lib().gtf_pfock_set_prim_tol(self.h, float(tol)), "gtf_pfock_set_prim_tol")
+
+    def prim_census(self):
+        """Per class: primitive quartets the kernels evaluate (FP64 tiles, FP32 tiles)."""
+        hi, lo = np.zeros(21), np.zeros(21)
+        if lib().gtf_pfock_prim_census(self.h, _p(hi), _p(lo)) != 21:
+            raise GtfError("gtf_pfock_prim_census failed")
+        return hi, lo
 
     def mixed_counts(self):
         nq = np.zeros(21, dtype=np.int64)
@@ -602,6 +616,9 @@ class Batch:
 
     def set_precision(self, tau):
         self._st(lib().gtf_batch_set_precision(self.h, tau), "gtf_batch_set_precision")
+
+    def set_prim_tol(self, tol):
+        self._st(lib().gtf_batch_set_prim_tol(self.h, float(tol)), "gtf_batch_set_prim_tol")
 
     def use_graph(self, on):
         self._st(lib().gtf_batch_use_graph(self.h, 1 if on else 0), "gtf_batch_use_graph")

@@ -771,6 +771,14 @@ int gtf_batch_set_precision(gtf_batch_t b, double tau)
     return b->sys.build_tiles();
 }
 
+int gtf_batch_set_prim_tol(gtf_batch_t b, double tol)
+{
+    if (!b || !(tol >= 0.0)) return -3;
+    b->sys.prim_tol = tol;
+    b->invalidate_graph();
+    return 0;
+}
+
 int gtf_batch_autotune(gtf_batch_t b, int reps, double* ms)
 {
     if (!b || reps <= 0) return -3;

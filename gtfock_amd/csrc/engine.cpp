@@ -168,6 +168,12 @@ int GtfSystem::init(const std::vector<const BasisSet*>& mols, hipStream_t st)
                 nz += (long)h_pnprim[id] * herm_nnz_lookup(la, lb);
             }
     }
+    for (int id = 0; id < npairs; id++)
+        if (h_pnprim[id] > GTF_MAX_RAW_PRIM_PAIRS) {
+            fprintf(stderr, "gtfock_amd: a shell pair has %d primitive pairs (at most %d supported)\n", h_pnprim[id],
+                    GTF_MAX_RAW_PRIM_PAIRS);
+            return -3;
+        }
     nprimpairs = np;
     nE = ne;
     d_s1 = dalloc<int>(npairs);
@@ -176,18 +182,21 @@ int GtfSystem::init(const std::vector<const BasisSet*>& mols, hipStream_t st)
     d_ppoff = dalloc<int>(npairs);
     d_peoff = dalloc<long>(npairs);
     d_prim = dalloc<double4>(np);
+    d_pq = dalloc<double>(np);
     d_E = dalloc<double>(ne);
     d_pzoff = dalloc<long>(npairs);
     d_Ez = dalloc<double>(nz);
     d_counters = dalloc<int>(2 * GTF_NCLASSES);
-    if (!d_s1 || !d_s2 || !d_pnprim || !d_ppoff || !d_peoff || !d_prim || !d_E || !d_counters || !d_pzoff || !d_Ez)
+    if (!d_s1 || !d_s2 || !d_pnprim || !d_ppoff || !d_peoff || !d_prim || !d_E || !d_counters || !d_pzoff || !d_Ez ||
+        !d_pq)
         return -2;
     if (upload(d_s1, h_s1, st) || upload(d_s2, h_s2, st) || upload(d_pnprim, h_pnprim, st) ||
         upload(d_ppoff, ppoff, st) || upload(d_peoff, peoff, st) || upload(d_pzoff, pzoff, st))
         return -1;
     BasisView bv{nshells, nbf, natoms, d_L, d_nprim, d_poff, d_fstart, d_atom,
                  d_alpha, d_coef, d_center, d_atom_xyz, d_atom_Z};
-    if (gtf_build_pairs(st, bv, npairs, d_s1, d_s2, d_ppoff, d_peoff, d_prim, d_E, d_pzoff, d_Ez, d_pnprim, prim_eps))
+    if (gtf_build_pairs(st, bv, npairs, d_s1, d_s2, d_ppoff, d_peoff, d_prim, d_E, d_pzoff, d_Ez, d_pnprim, d_pq,
+                        prim_eps))
         return -1;
     // surviving primitive pairs per shell pair
     SYS_CHECK(hipMemcpyAsync(h_pnprim.data(), d_pnprim, sizeof(int) * npairs, hipMemcpyDeviceToHost, st));
@@ -217,7 +226,7 @@ void GtfSystem::dfree(void* p)
 
 PairView GtfSystem::pair_view() const
 {
-    return PairView{d_s1, d_s2, d_pnprim, d_ppoff, d_peoff, d_prim, d_E, d_pzoff, d_Ez};
+    return PairView{d_s1, d_s2, d_pnprim, d_ppoff, d_peoff, d_prim, d_E, d_pzoff, d_Ez, d_pq, 0.0};
 }
 
 static int launch(eri_kernel_t fn, EriArgs& a, int blocks, hipStream_t st, size_t lds = 0)
@@ -513,6 +522,88 @@ long GtfSystem::quartets(int* out, long cap) const
     return k;
 }
 
+int GtfSystem::prim_census(double* hi, double* lo)
+{
+    SYS_CHECK(hipStreamSynchronize(stream));
+    unsigned long long* d_out = dalloc<unsigned long long>(2 * GTF_NCLASSES);
+    if (!d_out) return -2;
+    SYS_CHECK(hipMemsetAsync(d_out, 0, sizeof(unsigned long long) * 2 * GTF_NCLASSES, stream));
+    PairView pv = pair_view();
+    pv.peps = prim_tol;
+    std::vector<std::vector<int4>> rows;  // tile lists of the classes that run without tiles
+    std::vector<int4*> tmp;
+    int rc = 0;
+    for (int cls = 0; cls < GTF_NCLASSES && rc == 0; cls++) {
+        if (nq[cls] == 0) continue;
+        int A, B;
+        gtf_class_types(cls, A, B);
+        const EriClassInfo* ci = gtf_eri_class(cls);
+        CensusArgs c{};
+        c.bra_list = d_sig[A];
+        c.ket_list = d_sig[B];
+        c.vbra = d_sigv[A];
+        c.vket = d_sigv[B];
+        c.rank_bra = d_rank[A];
+        c.rank_ket = d_rank[B];
+        c.tol2 = tol2;
+        c.same = A == B;
+        c.screened = prim_screened_l(ci->la + ci->lb + ci->lc + ci->ld) ? 1 : 0;
+        c.pv = pv;
+        if (ntiles[cls] > 0 || ntiles_lo[cls] > 0) {
+            c.tau2 = class_tau2(cls);
+            if (ntiles[cls] > 0) {
+                c.tiles = d_tiles[cls];
+                c.ntiles = ntiles[cls];
+                c.low = 0;
+                c.out = d_out + cls;
+                if (gtf_prim_census(stream, c)) rc = -1;
+            }
+            if (ntiles_lo[cls] > 0 && rc == 0) {
+                c.tiles = d_tiles_lo[cls];
+                c.ntiles = ntiles_lo[cls];
+                c.low = 1;
+                c.out = d_out + GTF_NCLASSES + cls;
+                if (gtf_prim_census(stream, c)) rc = -1;
+            }
+        } else {
+            // the global-atomic path evaluates each row's exact ket prefix in FP64
+            rows.emplace_back();
+            std::vector<int4>& v = rows.back();
+            for (int k = 0; k < nrows[cls]; k++)
+                if (h_exact[cls][k] > 0) {
+                    const int s0 = seg_start[B][h_row_seg[cls][k]];
+                    v.push_back(make_int4(h_row_bra[cls][k], s0, s0 + (int)h_exact[cls][k], 1));
+                }
+            if (v.empty()) continue;
+            int4* d = dalloc<int4>(v.size());
+            if (!d || upload(d, v, stream)) {
+                rc = -1;
+                break;
+            }
+            tmp.push_back(d);
+            c.tiles = d;
+            c.ntiles = (int)v.size();
+            c.tau2 = 0.0;
+            c.low = 0;
+            c.out = d_out + cls;
+            if (gtf_prim_census(stream, c)) rc = -1;
+        }
+    }
+    unsigned long long h[2 * GTF_NCLASSES] = {0};
+    if (rc == 0 && (hipMemcpyAsync(h, d_out, sizeof h, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+                    hipStreamSynchronize(stream) != hipSuccess))
+        rc = -1;
+    (void)hipStreamSynchronize(stream);  // the row lists are read by the uploads
+    for (int4* d : tmp) dfree(d);
+    dfree(d_out);
+    if (rc) return rc;
+    for (int cls = 0; cls < GTF_NCLASSES; cls++) {
+        hi[cls] = (double)h[cls];
+        lo[cls] = (double)h[GTF_NCLASSES + cls];
+    }
+    return 0;
+}
+
 void GtfSystem::csr(std::vector<int>& shellptr, std::vector<int>& shellid, std::vector<int>& shellrid,
                     std::vector<double>& shellvalue) const
 {
@@ -800,6 +891,7 @@ int GtfSystem::launch_class(int cls, hipStream_t cs, const double* D, double* J,
     a.mol_active = d_mol_active;
     a.row_bra = d_row_bra[cls];
     a.row_kstart = d_row_kstart[cls];
+    a.pv.peps = prim_tol;
     if (ntiles[cls] > 0 || ntiles_lo[cls] > 0) {
         a.nbf = nbf;
         a.tau2 = class_tau2(cls);

@@ -85,9 +85,19 @@ __host__ __device__ constexpr int default_nc()
     return best;
 }
 
+// classes whose kernels apply the primitive screening of eri_block (by
+// total angular momentum L)
+__host__ __device__ constexpr bool prim_screened_l(int L) { return L <= 3; }
+template <int LA, int LB, int LC, int LD>
+__host__ __device__ constexpr bool prim_screened()
+{
+    return prim_screened_l(LA + LB + LC + LD);
+}
+
 // Contracted integrals (bra components c0..c0+NC-1, all ket components) of
 // the quartet (bra pair | ket pair), summed over the bra primitive pairs
-// [i0, i1) (i1 < 0: all).  acc[c][cd], cd = c_sph * nsph(LD) + d_sph.
+// [i0, i1) (i1 < 0: all) and, in the classes of prim_screened, the
+// primitive quartets whose Schwarz bound q_bra q_ket is at least pv.peps.  acc[c][cd], cd = c_sph * nsph(LD) + d_sph.
 // The ket coefficients are read in the compact nonzero-only layout and the
 // structurally zero terms are skipped at compile time; when a thread owns the
 // whole bra pair (NC == NAB, c0 == 0) the bra side is treated the same way.
@@ -111,8 +121,30 @@ __device__ __forceinline__ void eri_block(const PairView& pv, int bra, int ket, 
     const double4* __restrict__ pk = pv.prim + pv.poff[ket];
     const double* __restrict__ Eb = BRA_FULL ? pv.Ez + pv.zoff[bra] : pv.E + pv.eoff[bra] + (long)c0 * HAB;
     const double* __restrict__ Ek0 = pv.Ez + pv.zoff[ket];
+    // primitive screening (classes of total angular momentum <= 3, whose
+    // s and p shells carry the long contractions; the higher classes would
+    // pay for it in registers): a pair's primitives are stored by
+    // descending Schwarz bound, so the kets of bra primitive i that pass
+    // q_i q_j >= peps are a prefix, found by bisection once per i, and once
+    // one bra primitive has none the later ones have none either
+    constexpr bool PSCREEN = prim_screened<LA, LB, LC, LD>();
     const int ie = i1 < 0 ? nb : i1;
     for (int i = i0; i < ie; ++i) {
+        int nke = nk;
+        if constexpr (PSCREEN) {
+            if (pv.peps > 0.0) {
+                const double qi = pv.q[pv.poff[bra] + i];
+                const double* __restrict__ qk = pv.q + pv.poff[ket];
+                int lo = 0, hi = nk;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (qi * qk[mid] >= pv.peps) lo = mid + 1;
+                    else hi = mid;
+                }
+                nke = lo;
+                if (nke == 0) break;
+            }
+        }
         const double4 B = pb[i];
         Real eb[NC][HAB];
         static_for<0, NC>([&](auto c_) {
@@ -126,7 +158,7 @@ __device__ __forceinline__ void eri_block(const PairView& pv, int bra, int ket, 
                 }
             });
         });
-        for (int j = 0; j < nk; ++j) {
+        for (int j = 0; j < nke; ++j) {
             const double4 K = pk[j];
             // T = alpha |PQ|^2 and F_0..F_L(T) in FP64 for both precisions;
             // an FP32 tile rounds them for the R recursion and contractions

@@ -124,6 +124,8 @@ struct PairView {
     const double* E;     // Hermite->spherical coefficients, scaled by c_i c_j K_ab / p
     const long* zoff;    // offset into Ez[]
     const double* Ez;    // the same, structural nonzeros only (herm_nz_pos order)
+    const double* q;     // Schwarz bound per primitive pair (a pair's primitives in descending order)
+    double peps;         // primitive quartets with q_bra q_ket < peps are skipped (0: none)
 };
 
 struct BasisView {

@@ -617,12 +617,25 @@ int gtf_pfock_class_work(PFock_t pf, long* nquartets, double* nprimquartets)
     return GTF_NCLASSES;
 }
 
+int gtf_pfock_prim_census(PFock_t pf, double* npq_fp64, double* npq_fp32)
+{
+    if (!pf || !npq_fp64 || !npq_fp32) return -1;
+    return pf->sys.prim_census(npq_fp64, npq_fp32) ? -1 : GTF_NCLASSES;
+}
+
 int gtf_pfock_set_precision(PFock_t pf, double tau)
 {
     if (tau < 0.0) return -1;
     if (hipStreamSynchronize(pf->stream) != hipSuccess) return -1;
     pf->sys.tau = tau;
     return pf->sys.build_tiles();
+}
+
+int gtf_pfock_set_prim_tol(PFock_t pf, double tol)
+{
+    if (!pf || !(tol >= 0.0)) return -1;
+    pf->sys.prim_tol = tol;
+    return 0;
 }
 
 int gtf_pfock_mixed_counts(PFock_t pf, long* nq_lo, double* nprimq_lo)

@@ -2,9 +2,32 @@
 #pragma once
 #include "gtf_internal.h"
 
+// primitive pairs of every shell pair (p, P, Hermite blocks), the kept ones
+// in descending order of their Schwarz bounds qout; at most BP_MAXRAW
+// primitive pairs per shell pair (the caller checks)
+#define GTF_MAX_RAW_PRIM_PAIRS 512
 int gtf_build_pairs(hipStream_t st, const BasisView& b, int npairs, const int* s1, const int* s2,
                     const int* poff, const long* eoff, double4* prim, double* E, const long* zoff, double* Ez,
-                    int* nprim_out, double prim_eps);
+                    int* nprim_out, double* qout, double prim_eps);
+// primitive quartets evaluated over a tile list (prim_census_kernel)
+struct CensusArgs {
+    const int4* tiles;
+    int ntiles;
+    const int* bra_list;
+    const int* ket_list;
+    const double* vbra;
+    const double* vket;
+    const int* rank_bra;
+    const int* rank_ket;
+    double tol2;
+    int same;
+    double tau2;  // precision split: FP64 tiles keep v_bra v_ket >= tau2, FP32 tiles the rest
+    int low;
+    int screened;  // the class applies primitive screening
+    PairView pv;   // pv.peps: the threshold
+    unsigned long long* out;
+};
+int gtf_prim_census(hipStream_t st, const CensusArgs& c);
 int gtf_count_rows(hipStream_t st, int nrows, const int* row_bra, const int* row_seg, const double* vbra,
                    const int* rank_bra, const double* vket_sorted, const int* rank_sorted, const int* seg_start,
                    const int* seg_len, double tol2, int same, long* exact, long* counts, long* row_off);

@@ -12,44 +12,60 @@
 #include "screen.h"
 
 // Primitive pairs whose largest scaled coefficient |E| is below prim_eps are
-// dropped (their quartets contribute < 1e-15 to any Fock element); the
-// survivors are packed at the front of the pair's slots, in primitive order,
-// and nprim_out receives their count.
+// dropped (their quartets contribute < 1e-15 to any Fock element).  The
+// survivors are stored in descending order of their Schwarz bound
+//   q = sqrt(max over component pairs ab of (ab|ab)),
+// the largest self-repulsion of the primitive pair's product functions (the
+// same component pair on both sides, P = Q): any primitive quartet of the
+// pair obeys |(ab|cd)| <= q_bra q_ket (Schwarz inequality), and with the
+// bounds descending the ERI kernels end their primitive loops at the first
+// pair of bounds whose product is below the primitive threshold.  Ties keep
+// primitive order.  nprim_out receives the survivor count, qout the bounds.
 //
 // One 64-lane workgroup per shell pair.  Per shell pair the lanes tabulate,
 // in LDS, the Hermite index triples, the cartesian terms of every spherical
 // component pair and the compact position of every structurally nonzero
 // (component pair, Hermite index) entry.  Then for each primitive pair in
 // turn three lanes run the one-dimensional recursions E^x, E^y, E^z into LDS
-// and all lanes form the coefficient block from them, one entry per lane;
-// a wavefront max decides whether the primitive pair is kept.
+// and all lanes form the coefficient block from them, one entry per lane; a
+// wavefront max decides whether the primitive pair is kept.  A first pass
+// over the primitive pairs finds the kept ones and their bounds, the lanes
+// rank them, and a second pass forms the kept blocks again (the same
+// arithmetic, so the same values) and writes each at its rank.
 #define BP_MAXAB (nsph(GTF_LMAX) * nsph(GTF_LMAX))
 #define BP_MAXH nherm(2 * GTF_LMAX)
 #define BP_MAXTERM 9  // cartesian pairs of one spherical pair (d: at most 3 x 3)
 #define BP_NT (2 * GTF_LMAX + 2)
+#define BP_RL (4 * GTF_LMAX)  // highest Hermite order of a self-repulsion
+#define BP_MAXRAW GTF_MAX_RAW_PRIM_PAIRS  // primitive pairs of one shell pair
+#define BP_TWO_PI_2_5 34.986836655249725693  // 2 pi^(5/2)
 
 __global__ __launch_bounds__(64) void build_pairs_kernel(BasisView b, int npairs, const int* __restrict__ s1,
                                                          const int* __restrict__ s2, const int* __restrict__ poff,
                                                          const long* __restrict__ eoff, double4* __restrict__ prim,
                                                          double* __restrict__ E, const long* __restrict__ zoff,
                                                          double* __restrict__ Ez, int* __restrict__ nprim_out,
-                                                         double prim_eps)
+                                                         double* __restrict__ qout, double prim_eps)
 {
     __shared__ double e1[3][GTF_LMAX + 1][GTF_LMAX + 1][BP_NT];
     __shared__ double ent[BP_MAXAB * BP_MAXH];
     __shared__ int zpos[BP_MAXAB * BP_MAXH];
-    __shared__ int hcode[BP_MAXH];
+    __shared__ int hcode[nherm(BP_RL)];
     __shared__ double tcoef[BP_MAXAB][BP_MAXTERM];
     __shared__ int tcode[BP_MAXAB][BP_MAXTERM];
     __shared__ int nterm[BP_MAXAB];
     __shared__ double pp[5];  // p, P, scale of the current primitive pair
+    __shared__ double rz[nherm(BP_RL)];  // R_tuv(p / 2, 0) of the current primitive pair
+    __shared__ double qraw[BP_MAXRAW];   // bound of each primitive pair, -1 if dropped
+    __shared__ int rnk[BP_MAXRAW];
     const int pid = blockIdx.x, lane = threadIdx.x;
     const int A = s1[pid], B = s2[pid];
     const int la = b.L[A], lb = b.L[B];
     const int na = nsph(la), nb = nsph(lb), H = nherm(la + lb), nab = na * nb, esz = nab * H;
     const int nA = b.nprim[A], nB = b.nprim[B], nraw = nA * nB;
+    const int L2 = 2 * (la + lb), HR = nherm(L2);
     // ---- per shell pair tables
-    for (int h = lane; h < H; h += 64) hcode[h] = herm_t(h) | herm_u(h) << 4 | herm_v(h) << 8;
+    for (int h = lane; h < HR; h += 64) hcode[h] = herm_t(h) | herm_u(h) << 4 | herm_v(h) << 8;
     const int ca0 = la == 0 ? 0 : nherm(la - 1), cb0 = lb == 0 ? 0 : nherm(lb - 1);
     for (int ab = lane; ab < nab; ab += 64) {
         const int as = ab / nb, bs = ab - (ab / nb) * nb;
@@ -83,9 +99,9 @@ __global__ __launch_bounds__(64) void build_pairs_kernel(BasisView b, int npairs
     const double AB2 = (CA[0] - CB[0]) * (CA[0] - CB[0]) + (CA[1] - CB[1]) * (CA[1] - CB[1]) +
                        (CA[2] - CB[2]) * (CA[2] - CB[2]);
     __syncthreads();
-    // ---- primitive pairs
-    int kept = 0;
-    for (int r = 0; r < nraw; r++) {
+    // the coefficient block of primitive pair r into ent (and p, P, scale
+    // into pp); returns the wavefront max of |E|, the same on every lane
+    auto block = [&](int r) {
         const int i = r / nB, j = r - (r / nB) * nB;
         const double a = b.alpha[b.poff[A] + i], bb = b.alpha[b.poff[B] + j];
         const double p = a + bb;
@@ -131,15 +147,94 @@ __global__ __launch_bounds__(64) void build_pairs_kernel(BasisView b, int npairs
             lmax = fmax(lmax, fabs(val));
         }
         for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
+        return lmax;
+    };
+    // ---- pass 1: which primitive pairs are kept, and their bounds
+    for (int r = 0; r < nraw; r++) {
+        const double lmax = block(r);
+        double q = -1.0;
         if (lmax >= prim_eps) {  // the same on every lane
-            double* Eo = E + eoff[pid] + (long)kept * esz;
-            double* Zo = Ez + zoff[pid] + (long)kept * ez;
-            for (int idx = lane; idx < esz; idx += 64) {
-                Eo[idx] = ent[idx];
-                if (zpos[idx] >= 0) Zo[zpos[idx]] = ent[idx];
+            const double p = pp[0];
+            // R_tuv(alpha = p / 2, 0) for t + u + v <= L2, stage by stage as
+            // in hermite_R (the X R terms vanish at zero distance;
+            // F_n(0) = 1 / (2n + 1))
+            if (lane == 0) {
+                const double m2a = -p;  // -2 alpha
+                double pw = 1.0;
+                for (int n = 0; n < L2; n++) pw *= m2a;
+                rz[0] = pw / (2 * L2 + 1);
+                for (int n = L2 - 1; n >= 0; n--) {
+                    pw /= m2a;
+                    for (int h = nherm(L2 - n) - 1; h >= 1; h--) {
+                        const int hc = hcode[h];
+                        const int t = hc & 15, u = (hc >> 4) & 15, v = hc >> 8;
+                        double rr;
+                        if (t > 0) rr = t > 1 ? (t - 1) * rz[hidx(t - 2, u, v)] : 0.0;
+                        else if (u > 0) rr = u > 1 ? (u - 1) * rz[hidx(t, u - 2, v)] : 0.0;
+                        else rr = v > 1 ? (v - 1) * rz[hidx(t, u, v - 2)] : 0.0;
+                        rz[h] = rr;
+                    }
+                    rz[0] = pw / (2 * n + 1);
+                }
             }
-            if (lane == 0) prim[poff[pid] + kept] = make_double4(pp[0], pp[1], pp[2], pp[3]);
-            kept++;
+            __syncthreads();
+            // (ab|ab) / prefactor, one component pair per lane
+            double dmax = 0.0;
+            for (int ab = lane; ab < nab; ab += 64) {
+                const double* eab = ent + ab * H;
+                double d = 0.0;
+                for (int h1 = 0; h1 < H; h1++) {
+                    const double x = eab[h1];
+                    if (x == 0.0) continue;
+                    const int c1 = hcode[h1];
+                    double sm = 0.0;
+                    for (int h2 = 0; h2 < H; h2++) {
+                        const double y = eab[h2];
+                        if (y == 0.0) continue;
+                        const int c2 = hcode[h2];
+                        const int t2 = c2 & 15, u2 = (c2 >> 4) & 15, v2 = c2 >> 8;
+                        const double rv = rz[hidx((c1 & 15) + t2, ((c1 >> 4) & 15) + u2, (c1 >> 8) + v2)];
+                        sm += ((t2 + u2 + v2) & 1) ? -y * rv : y * rv;
+                    }
+                    d += x * sm;
+                }
+                dmax = fmax(dmax, d);
+            }
+            for (int off = 32; off > 0; off >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, off));
+            q = sqrt(BP_TWO_PI_2_5 * dmax / sqrt(2.0 * p));
+        }
+        if (lane == 0) qraw[r] = q;
+        __syncthreads();  // e1, pp, ent and rz are rewritten for the next primitive pair
+    }
+    // ---- ranks of the kept primitive pairs by descending bound
+    int kept = 0;
+    for (int r = lane; r < nraw; r += 64) {
+        const double q = qraw[r];
+        int k = 0;
+        if (q >= 0.0)
+            for (int x = 0; x < nraw; x++) {
+                const double qx = qraw[x];
+                k += qx > q || (qx == q && x < r);
+            }
+        rnk[r] = k;
+        kept += q >= 0.0;
+    }
+    for (int off = 32; off > 0; off >>= 1) kept += __shfl_xor(kept, off);
+    __syncthreads();
+    // ---- pass 2: the kept blocks at their ranks
+    for (int r = 0; r < nraw; r++) {
+        if (qraw[r] < 0.0) continue;  // the same on every lane
+        block(r);
+        const int k = rnk[r];
+        double* Eo = E + eoff[pid] + (long)k * esz;
+        double* Zo = Ez + zoff[pid] + (long)k * ez;
+        for (int idx = lane; idx < esz; idx += 64) {
+            Eo[idx] = ent[idx];
+            if (zpos[idx] >= 0) Zo[zpos[idx]] = ent[idx];
+        }
+        if (lane == 0) {
+            prim[poff[pid] + k] = make_double4(pp[0], pp[1], pp[2], pp[3]);
+            qout[poff[pid] + k] = qraw[r];
         }
         __syncthreads();  // e1 and pp are rewritten for the next primitive pair
     }
@@ -221,14 +316,57 @@ __global__ void fill_qlist_diag_kernel(int n, const int* __restrict__ ids, int2*
     if (k < n) q[k] = make_int2(ids[k], ids[k]);
 }
 
+// Primitive quartets the J/K kernels evaluate over a list of tiles (bra
+// row, first ket, end ket): for each quartet of a tile that passes the
+// kernels' tests (Schwarz, one orientation in a diagonal class, the
+// precision split), the pairs (i, j) of bra and ket primitives with
+// q_i q_j >= peps when the class screens primitives, else all of them.
+// With both lists in descending order of q that is what the kernels' loops
+// visit, whichever pair they take as the bra.  One workgroup per tile, one
+// thread per ket; the total is added to *out.
+__global__ __launch_bounds__(256) void prim_census_kernel(CensusArgs c)
+{
+    const int4 t = c.tiles[blockIdx.x];
+    const int bra = c.bra_list[t.x];
+    const int nbp = c.pv.nprim[bra];
+    const double* __restrict__ qb = c.pv.q + c.pv.poff[bra];
+    const double vb = c.vbra[t.x];
+    unsigned long long cnt = 0;
+    for (int j = t.y + threadIdx.x; j < t.z; j += blockDim.x) {
+        if (!(vb * c.vket[j] >= c.tol2) || (c.same && c.rank_ket[j] > c.rank_bra[t.x])) continue;
+        if ((vb * c.vket[j] >= c.tau2) == (c.low != 0)) continue;
+        const int ket = c.ket_list[j];
+        const int nk = c.pv.nprim[ket];
+        if (!c.screened || !(c.pv.peps > 0.0)) {
+            cnt += (unsigned long long)nbp * nk;
+            continue;
+        }
+        const double* __restrict__ qk = c.pv.q + c.pv.poff[ket];
+        for (int i = 0; i < nbp; i++) {
+            const double qi = qb[i];
+            int lo = 0, hi = nk;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (qi * qk[mid] >= c.pv.peps) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo == 0) break;
+            cnt += lo;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(c.out, cnt);
+}
+
 // ---- host launchers -------------------------------------------------------
 
 int gtf_build_pairs(hipStream_t st, const BasisView& b, int npairs, const int* s1, const int* s2,
                     const int* poff, const long* eoff, double4* prim, double* E, const long* zoff, double* Ez,
-                    int* nprim_out, double prim_eps)
+                    int* nprim_out, double* qout, double prim_eps)
 {
     if (npairs <= 0) return 0;
-    build_pairs_kernel<<<npairs, 64, 0, st>>>(b, npairs, s1, s2, poff, eoff, prim, E, zoff, Ez, nprim_out, prim_eps);
+    build_pairs_kernel<<<npairs, 64, 0, st>>>(b, npairs, s1, s2, poff, eoff, prim, E, zoff, Ez, nprim_out, qout,
+                                              prim_eps);
     GTF_CHECK(hipGetLastError());
     return 0;
 }
@@ -250,6 +388,14 @@ int gtf_fill_qlist_diag(hipStream_t st, int n, const int* ids, int2* q)
 {
     if (n <= 0) return 0;
     fill_qlist_diag_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, ids, q);
+    GTF_CHECK(hipGetLastError());
+    return 0;
+}
+
+int gtf_prim_census(hipStream_t st, const CensusArgs& c)
+{
+    if (c.ntiles <= 0) return 0;
+    prim_census_kernel<<<c.ntiles, 256, 0, st>>>(c);
     GTF_CHECK(hipGetLastError());
     return 0;
 }

@@ -67,6 +67,18 @@ int gtf_pfock_class_work(PFock_t pfock, long *nquartets, double *nprimquartets);
 int gtf_pfock_set_precision(PFock_t pfock, double tau);
 /* Per class: quartets and primitive quartets evaluated in FP32. */
 int gtf_pfock_mixed_counts(PFock_t pfock, long *nquartets_fp32, double *nprimquartets_fp32);
+/* Primitive screening of the J/K builds (the role of ERD's internal
+ * primitive screening under pfock/fock_task.c:197-207): primitive quartets
+ * whose Schwarz bound sqrt((ij|ij)(kl|kl)) is below tol are skipped, in
+ * the quartet classes of total angular momentum <= 3 (whose s and p shells
+ * carry the long contractions); 0 evaluates every primitive quartet of a
+ * screened quartet.  Default 1e-15.  Takes effect from the next
+ * PFock_computeFock.  Returns 0, or -1 for tol < 0. */
+int gtf_pfock_set_prim_tol(PFock_t pfock, double tol);
+/* Per class (21): primitive quartets the J/K kernels evaluate at the
+ * current threshold, in FP64 and in FP32 tiles (with tol = 0, FP64 + FP32
+ * = the primitive quartets of gtf_pfock_class_work).  Returns 21, or -1. */
+int gtf_pfock_prim_census(PFock_t pfock, double *npq_fp64, double *npq_fp32);
 
 /* Per-class kernel timing with HIP events recorded on the engine's stream
  * around each class launch; enabling resets the accumulators. */
@@ -188,6 +200,9 @@ int gtf_batch_layout(gtf_batch_t batch, int *nbf, long *offset);
 long gtf_batch_total(gtf_batch_t batch);
 /* FP32 ERIs for quartets with Schwarz bound below tau (0: FP64 only) */
 int gtf_batch_set_precision(gtf_batch_t batch, double tau);
+/* primitive screening threshold of the batch's builds, as
+ * gtf_pfock_set_prim_tol */
+int gtf_batch_set_prim_tol(gtf_batch_t batch, double tol);
 /* mixed-precision purification inside gtf_batch_scf, per molecule, as
  * gtf_purif_set_precision (mode 0 FP64, 1 FP32, 2 split BF16) */
 int gtf_batch_set_purif_precision(gtf_batch_t batch, int mode, double until);

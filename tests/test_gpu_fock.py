@@ -140,6 +140,62 @@ def test_fock_c10h22_matches_oracle(g, gbs, mol_dir):
 
 
 @gpu
+def test_primitive_screening_against_oracle(g, gbs, mol_dir):
+    """Primitive screening (default 1e-15) against the oracle, which
+    evaluates every primitive quartet: the default stays two orders inside
+    the Fock bar, a higher threshold skips more and errs more (each skipped
+    primitive quartet is below its Schwarz bound), 0 turns it off, and a
+    negative threshold is refused."""
+    xyz = str(mol_dir / "butane.xyz")
+    b = g.BasisSet(gbs, xyz)
+    ob = oracle.Basis(gbs, xyz)
+    D = _rand_sym(b.nbf, 5)
+    Jr, Kr, _ = ob.fock(D)
+    pf = g.PFock(b, tolscr=1e-11)
+    err = {}
+    for tol in (None, 0.0, 1e-15, 1e-12, 1e-9):
+        if tol is not None:
+            pf.set_prim_tol(tol)
+        err[tol] = np.abs(pf.fock(D) - (Jr + Kr)).max()
+        print("prim_tol %s: max|dF| = %.3e" % (tol, err[tol]))
+    assert err[None] == err[1e-15] or abs(err[None] - err[1e-15]) < 1e-13
+    assert err[0.0] < FOCK_TOL
+    assert err[1e-15] < FOCK_TOL / 100
+    assert err[1e-9] > err[1e-15]
+    assert err[1e-9] < 1e-5
+    with pytest.raises(g.GtfError):
+        pf.set_prim_tol(-1.0)
+    pf.close()
+
+
+@gpu
+def test_primitive_census(g, gbs, mol_dir):
+    """The census of evaluated primitive quartets: with no primitive
+    threshold it equals the screened quartets' primitive quartets class by
+    class (FP64 and FP32 tiles together, with mixed precision too); with the
+    default threshold it is smaller in the screened classes (total angular
+    momentum <= 3) and unchanged in the others."""
+    b = g.BasisSet(gbs, str(mol_dir / "c10h22.xyz"))
+    pf = g.PFock(b)
+    nq, npq = pf.class_work()
+    names = g.class_names()
+    low_l = np.array([sum({"s": 0, "p": 1, "d": 2}[ch] for ch in nm if ch in "spd") <= 3 for nm in names])
+    pf.set_prim_tol(0.0)
+    hi, lo = pf.prim_census()
+    assert np.array_equal(hi + lo, npq) and not lo.any()
+    pf.set_precision(1e-4)
+    hi, lo = pf.prim_census()
+    assert np.array_equal(hi + lo, npq) and lo.sum() > 0
+    pf.set_precision(0.0)
+    pf.set_prim_tol(1e-15)
+    hi, lo = pf.prim_census()
+    print("evaluated / all primitive quartets: %.4f" % (hi.sum() / npq.sum()))
+    assert np.array_equal(hi[~low_l], npq[~low_l])
+    assert (hi[low_l] <= npq[low_l]).all() and hi[low_l].sum() < npq[low_l].sum()
+    pf.close()
+
+
+@gpu
 def test_one_electron_matches_oracle(g, gbs, mol_dir):
     xyz = str(mol_dir / "butane.xyz")
     b = g.BasisSet(gbs, xyz)
