@@ -562,6 +562,7 @@ int GtfSystem::prim_census(double* hi, double* lo)
                 c.tiles = d_tiles_lo[cls];
                 c.ntiles = ntiles_lo[cls];
                 c.low = 1;
+                c.pv.peps = prim_tol_fp32();
                 c.out = d_out + GTF_NCLASSES + cls;
                 if (gtf_prim_census(stream, c)) rc = -1;
             }
@@ -910,6 +911,7 @@ int GtfSystem::launch_class(int cls, hipStream_t cs, const double* D, double* J,
         if (ntiles_lo[cls] > 0) {
             a.tiles = d_tiles_lo[cls];
             a.ntiles = ntiles_lo[cls];
+            a.pv.peps = prim_tol_fp32();
             a.counter = d_counters + GTF_NCLASSES + cls;
             a.wg_stat = wg_record(2 * cls + 1, std::min(a.ntiles, grid));
             if (launch(exch ? ci->jk_tile_t_lo[bl] : ci->jk_tile_lo[bl], a, std::min(a.ntiles, grid), cs, lds))

@@ -62,6 +62,11 @@ struct GtfSystem {
     // 1e-15 changes C54H18 / cc-pVDZ Fock elements by 1.7e-12, within the
     // spread of two unscreened builds (FP64 atomics), and C10H22 by 1.6e-13
     double prim_tol = GTF_PRIM_TOL_DEFAULT;
+    // the threshold of the FP32 tiles: their quartets have Schwarz bound
+    // below tau and FP32 errors of about 1e-7 tau, so they skip primitive
+    // quartets below 1e-9 tau as well (1e-13 at tau = 1e-4); none when
+    // prim_tol = 0
+    double prim_tol_fp32() const { return prim_tol > 0.0 ? std::max(prim_tol, 1e-9 * tau) : 0.0; }
     // ---- Schwarz values and screening
     std::vector<double> h_v;
     std::vector<double> mol_vmax;

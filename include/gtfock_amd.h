@@ -72,8 +72,10 @@ int gtf_pfock_mixed_counts(PFock_t pfock, long *nquartets_fp32, double *nprimqua
  * whose Schwarz bound sqrt((ij|ij)(kl|kl)) is below tol are skipped, in
  * the quartet classes of total angular momentum <= 3 (whose s and p shells
  * carry the long contractions); 0 evaluates every primitive quartet of a
- * screened quartet.  Default 1e-15.  Takes effect from the next
- * PFock_computeFock.  Returns 0, or -1 for tol < 0. */
+ * screened quartet.  Default 1e-15.  With mixed precision the FP32 tiles
+ * use max(tol, 1e-9 tau) (their quartets are below tau and carry FP32
+ * errors of about 1e-7 tau), none with tol = 0.  Takes effect from the
+ * next PFock_computeFock.  Returns 0, or -1 for tol < 0. */
 int gtf_pfock_set_prim_tol(PFock_t pfock, double tol);
 /* Per class (21): primitive quartets the J/K kernels evaluate at the
  * current threshold, in FP64 and in FP32 tiles (with tol = 0, FP64 + FP32
