@@ -1082,7 +1082,8 @@ double gtf_batch_job_bytes(BasisSet_t basis)
             int la = b->L[s], lb = b->L[t];
             if (lb > la) std::swap(la, lb);
             const double np = (double)b->nprim[s] * b->nprim[t];
-            pairs += np * (32.0 + 8.0 * (nsph(la) * nsph(lb) * nherm(la + lb) + herm_nnz_lookup(la, lb))) + 96.0;
+            // per primitive pair: (p, P), its Schwarz bound, full and compact E
+            pairs += np * (40.0 + 8.0 * (nsph(la) * nsph(lb) * nherm(la + lb) + herm_nnz_lookup(la, lb))) + 96.0;
         }
     const double n = b->nbf, pl = ((b->nbf + 63) / 64) * 64;
     // + the split purification's BF16 slices of both operands at the most
