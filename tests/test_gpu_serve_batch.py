@@ -102,6 +102,34 @@ def test_batch_graph_replays_equal_eager(g, gbs, small_batch):
 
 
 @gpu
+def test_batch_primitive_threshold_reaches_the_graph(g, gbs, small_batch):
+    """gtf_batch_set_prim_tol takes effect in the captured graph (it is
+    captured again): 0 and the default agree to rounding, a high threshold
+    visibly changes G, and going back to 0 restores it."""
+    bases = [g.BasisSet(gbs, p) for p in small_batch]
+    Ds = [_rand_sym(b.nbf, 400 + i) for i, b in enumerate(bases)]
+    bt = g.Batch(bases)
+    G_def = bt.fock(Ds)
+    assert bt.stats()["graph"] == 1.0
+    bt.set_prim_tol(0.0)
+    G0 = bt.fock(Ds)
+    bt.set_prim_tol(1e-9)
+    G9 = bt.fock(Ds)
+    bt.set_prim_tol(0.0)
+    G0b = bt.fock(Ds)
+    assert bt.stats()["graph"] == 1.0
+    d_def = max(np.abs(x - y).max() for x, y in zip(G_def, G0))
+    d9 = max(np.abs(x - y).max() for x, y in zip(G9, G0))
+    d0 = max(np.abs(x - y).max() for x, y in zip(G0b, G0))
+    print("max|G - G(prim_tol 0)|: default %.2e, 1e-9 %.2e, 0 again %.2e" % (d_def, d9, d0))
+    assert d_def < 1e-12 and d0 < 1e-12
+    assert 1e-12 < d9 < 1e-5
+    with pytest.raises(g.GtfError):
+        bt.set_prim_tol(-1.0)
+    bt.close()
+
+
+@gpu
 def test_batch_scf_matches_single_scf(g, gbs, small_batch):
     bases = [g.BasisSet(gbs, p) for p in small_batch]
     bt = g.Batch(bases)
