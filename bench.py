@@ -589,6 +589,8 @@ def main():
     ap.add_argument("--serve-cohorts", type=int, default=16,
                     help="cohorts the server runs at once (16: a request starts as soon as it arrives at this load; "
                          "4 queued small requests behind flake cohorts, p50 9.6 s instead of 1.3-2.5 s)")
+    ap.add_argument("--serve-large-grid", type=int, default=0,
+                    help="cap the persistent workgroups per class launch of flake cohorts (0: as tuned)")
     ap.add_argument("--serve-large", type=int, default=2,
                     help="of those, cohorts of molecules with 384 or more basis functions")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
@@ -790,6 +792,10 @@ def main():
             for name, lo, hi in (("batch_mixed", 0, 1 << 30), ("c54h18_mixed", 384, 1 << 30)):
                 sh, _ = tuning.load(name)
                 if sh is not None:
+                    if lo > 0 and args.serve_large_grid > 0:
+                        # flake cohorts hold fewer persistent workgroups
+                        sh = [dict(x, grid=min(x["grid"], args.serve_large_grid)) for x in sh]
+                        name += " (grid <= %d)" % args.serve_large_grid
                     ranges.append((lo, hi, sh))
                     used.append("%s for nbf %d.." % (name, lo))
         served = serve_leg(g, scf_gbs, args.serve, args.serve_rate, 20261203, args.tau, args.serve_rings,

@@ -135,6 +135,7 @@ def lib():
         "gtf_server_set_large_slots": ([_vp, ctypes.c_int], ctypes.c_int),
         "gtf_server_set_purif_precision": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "gtf_server_set_join_gap": ([_vp, ctypes.c_int], ctypes.c_int),
+        "gtf_server_set_large_yield": ([_vp, ctypes.c_double], ctypes.c_int),
         "gtf_server_join_stats": ([_vp, _lp, _lp], ctypes.c_int),
         "gtf_batch_scf_begin": ([_vp, ctypes.c_int, ctypes.c_double, _vp, _vp], ctypes.c_int),
         "gtf_batch_scf_step": ([_vp], ctypes.c_int),
@@ -810,6 +811,13 @@ class Server:
         st = lib().gtf_server_set_join_gap(self.h, int(iters))
         if st != 0:
             raise GtfError("gtf_server_set_join_gap failed (%d)" % st)
+
+    def set_large_yield(self, factor):
+        """How long large-molecule cohorts defer an iteration while small ones
+        run, in units of their last iteration (0: never)."""
+        st = lib().gtf_server_set_large_yield(self.h, float(factor))
+        if st != 0:
+            raise GtfError("gtf_server_set_large_yield failed (%d)" % st)
 
     def join_stats(self):
         """(jobs admitted into running cohorts, regroups)"""

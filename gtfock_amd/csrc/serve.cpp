@@ -32,6 +32,8 @@
 #include <sys/time.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <climits>
 #include <condition_variable>
 #include <deque>
@@ -82,6 +84,11 @@ struct GtfServer {
     // cohorts of large molecules running at once, and their cap: half the
     // cohort slots, so smaller requests always find a slot
     int running_large = 0, max_large = 1;
+    // small molecules first: cohorts of small molecules running now, and how
+    // long a cohort of large molecules defers its next SCF iteration while
+    // there are any, in units of its own last iteration (0: never)
+    std::atomic<int> small_cohorts{0};
+    std::atomic<double> large_yield{2.0};
     bool stopping = false;
     std::thread scheduler;
     std::vector<std::thread> workers;
@@ -199,12 +206,27 @@ void GtfServer::run_cohort(std::vector<long> ids, double mem, int cohort, bool l
     }
     CohortCtx ctx{this, ids, 0.0};
     gtf_batch_t b = nullptr;
+    const bool small = cls < LARGE_CLASS;
+    if (small) small_cohorts++;
     int rc = open_batch(this, bases, sh, &ctx, &b);
     int step = 0, last_regroup = -1000000;
-    double regroup_s = 0.0;
+    double regroup_s = 0.0, last_iter = 0.0;
     const double t_begin = swtime();
     while (rc == 0) {
+        // A large cohort's persistent ERI kernels hold every CU for tens of
+        // milliseconds per class launch, so small cohorts beside it slow
+        // several-fold: it waits while small cohorts run, at most
+        // large_yield times its own last iteration (so it is never
+        // starved), and the small jobs, which finish in a few iterations,
+        // mostly have the device to themselves.
+        if (!small) {
+            const double cap = large_yield.load() * last_iter, t0 = swtime();
+            while (small_cohorts.load() > 0 && swtime() - t0 < cap)
+                std::this_thread::sleep_for(std::chrono::microseconds(500));
+        }
+        const double t_iter = swtime();
         const int left = gtf_batch_scf_step(b);
+        last_iter = swtime() - t_iter;
         if (left < 0) {
             rc = left;
             break;
@@ -303,6 +325,7 @@ void GtfServer::run_cohort(std::vector<long> ids, double mem, int cohort, bool l
         regroup_s += swtime() - t0;
         last_regroup = step;
     }
+    if (small) small_cohorts--;
     gtf_batch_destroy(b);
     std::lock_guard<std::mutex> lk(mu);
     const double t = swtime();
@@ -463,6 +486,13 @@ int gtf_server_set_purif_precision(gtf_server_t s, int mode, double until)
     std::lock_guard<std::mutex> lk(s->mu);
     s->purif_mode = mode;
     s->purif_until = until;
+    return 0;
+}
+
+int gtf_server_set_large_yield(gtf_server_t s, double factor)
+{
+    if (!s || !(factor >= 0.0)) return -3;
+    s->large_yield.store(factor);
     return 0;
 }
 

@@ -325,6 +325,13 @@ int gtf_server_set_purif_precision(gtf_server_t server, int mode, double until);
 /* SCF iterations between two admissions into a running cohort (default 1;
  * a negative value: cohorts never admit jobs while they run) */
 int gtf_server_set_join_gap(gtf_server_t server, int iters);
+/* Small molecules first: while cohorts of molecules below 384 basis
+ * functions run, a cohort of larger molecules waits before each SCF
+ * iteration for at most factor times its own last iteration (default 2;
+ * 0: never waits), so small jobs are not slowed by the large cohorts'
+ * ERI kernels and large ones are never starved.  Returns 0, or -3 for
+ * factor < 0. */
+int gtf_server_set_large_yield(gtf_server_t server, double factor);
 /* returns the job id (>= 0) or -3 */
 long gtf_server_submit(gtf_server_t server, BasisSet_t basis);
 /* status: 0 queued, 1 running, 2 done, < 0 failed (-2: larger than the

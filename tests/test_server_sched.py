@@ -326,3 +326,30 @@ def test_join_gap_off_keeps_cohorts_closed(lib):
     assert join_stats(L, h)[0] == 0
     assert int(poll(L, h, ids[0])[1][5]) != int(poll(L, h, ids[1])[1][5])
     L.gtf_server_destroy(h)
+
+
+def test_large_cohorts_yield_to_small_ones(lib):
+    """While small-molecule cohorts run, a large-molecule cohort waits before
+    each iteration for at most large_yield times its last iteration: with
+    the default (2) a large SCF shorter than the small ones ends after them,
+    with 0 it runs at full speed and ends first; it always completes."""
+    L = lib
+    L.gtf_server_set_large_yield.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    out = {}
+    for factor in (0.0, 2.0):
+        L.stub_reset()
+        h = server(L, 1e13, 4, 1)
+        assert L.gtf_server_set_large_yield(h, -1.0) == -3
+        assert L.gtf_server_set_large_yield(h, factor) == 0
+        t0 = time.time()
+        big = L.gtf_server_submit(h, L.stub_molecule(800, 0.2, -10.0))
+        small = [L.gtf_server_submit(h, L.stub_molecule(30, 0.4, -1.0)) for _ in range(2)]
+        L.gtf_server_wait(h)
+        st, r = poll(L, h, big)
+        assert st == 2
+        out[factor] = (r[4] - t0, max(poll(L, h, i)[1][4] for i in small) - t0)
+        L.gtf_server_destroy(h)
+    print(out)
+    assert out[0.0][0] < out[0.0][1]
+    assert out[2.0][0] > out[2.0][1]
+
